@@ -1,0 +1,76 @@
+/*
+ * libzmlhip device-resident API (MI355X / gfx950).
+ *
+ * Same operations as the CBLAS/LAPACKE exports (cblas.h, clapack.h) on
+ * device pointers, ordered on a caller-supplied HIP stream (`stream` is a
+ * hipStream_t passed as void*; NULL is the default stream). They return as
+ * soon as the work is queued; LAPACK info values are written to device
+ * memory. Return value: 0 when the work was queued, -i for an invalid
+ * argument i (matrix_layout counted as 1, as in clapack.h), or one of the
+ * ZMLHIP_E* codes below (message in zmlhip_last_error()).
+ *
+ * Enumerations: layout 101/102, trans 111/112/113, uplo 121/122,
+ * diag 131/132, side 141/142 (reference include/cblas.h:11-37); the LAPACK
+ * style routines take the LAPACKE characters ('N','T','C' / 'U','L').
+ */
+#ifndef ZMLHIP_H
+#define ZMLHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZMLHIP_EHIP (-1000)      /* a HIP runtime call failed */
+#define ZMLHIP_EALLOC (-1001)    /* device allocation failed */
+#define ZMLHIP_ESYNC (-1002)     /* a bounded device-side wait timed out */
+#define ZMLHIP_ENOTSUP (-1003)   /* shape or option not supported */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int zmlhip_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *zmlhip_last_error(void);
+/* Waits for all work queued on `stream`; returns 0 or ZMLHIP_EHIP. */
+int zmlhip_stream_synchronize(void *stream);
+
+/* ---- BLAS (device pointers) ------------------------------------------- */
+/* cblas_dgemm (src/cblas.zig:526; blas/gemm.zig:14-1033) */
+int zmlhip_dgemm(int layout, int transa, int transb, int m, int n, int k, double alpha,
+                 const double *a, int lda, const double *b, int ldb, double beta, double *c,
+                 int ldc, void *stream);
+/* cblas_dtrsm (src/cblas.zig:636; blas/trsm.zig:18-735) */
+int zmlhip_dtrsm(int layout, int side, int uplo, int transa, int diag, int m, int n, double alpha,
+                 const double *a, int lda, double *b, int ldb, void *stream);
+/* cblas_dsyrk (src/cblas.zig:587; blas/syrk.zig:15-366) */
+int zmlhip_dsyrk(int layout, int uplo, int trans, int n, int k, double alpha, const double *a,
+                 int lda, double beta, double *c, int ldc, void *stream);
+/* cblas_idamax (src/cblas.zig:247; blas/iamax.zig:11-68): writes the 0-based
+ * index to *result (device int). */
+int zmlhip_idamax(int n, const double *x, int incx, int *result, void *stream);
+/* cblas_dscal (src/cblas.zig:213; blas/scal.zig:9-41) */
+int zmlhip_dscal(int n, double alpha, double *x, int incx, void *stream);
+/* cblas_dswap (src/cblas.zig:234; blas/swap.zig) */
+int zmlhip_dswap(int n, double *x, int incx, double *y, int incy, void *stream);
+/* cblas_dgemv (src/cblas.zig:291; blas/gemv.zig:14-360) */
+int zmlhip_dgemv(int layout, int transa, int m, int n, double alpha, const double *a, int lda,
+                 const double *x, int incx, double beta, double *y, int incy, void *stream);
+
+/* ---- LAPACK (device pointers; ipiv and info are device memory) -------- */
+/* LAPACKE_dlaswp (src/linalg/lapack.zig:375; lapack/laswp.zig:12-95) */
+int zmlhip_dlaswp(int layout, int n, double *a, int lda, int k1, int k2, const int *ipiv,
+                  int incx, void *stream);
+/* LAPACKE_dgetrf (src/linalg/lapack.zig:1169; lapack/getrf.zig:14-152) */
+int zmlhip_dgetrf(int layout, int m, int n, double *a, int lda, int *ipiv, int *info,
+                  void *stream);
+/* LAPACKE_dgetrs (src/linalg/lapack.zig:1575; lapack/getrs.zig:13-142) */
+int zmlhip_dgetrs(int layout, char trans, int n, int nrhs, const double *a, int lda,
+                  const int *ipiv, double *b, int ldb, void *stream);
+/* LAPACKE_dpotrf (src/linalg/lapack.zig:3012; lapack/potrf.zig:17-240) */
+int zmlhip_dpotrf(int layout, char uplo, int n, double *a, int lda, int *info, void *stream);
+/* LAPACKE_dpotrs (src/linalg/lapack.zig:3466; lapack/potrs.zig:15-117) */
+int zmlhip_dpotrs(int layout, char uplo, int n, int nrhs, const double *a, int lda, double *b,
+                  int ldb, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -20,3 +20,18 @@ def oracle_lib():
         oracle.build(force=False)
     oracle.lib()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def zh():
+    """libzmlhip.so through its C ABI (fails loudly when it is not built)."""
+    from zml_amd import _lib
+    return _lib.load()
+
+
+@pytest.fixture(scope="session")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need the MI355X"
+    torch.cuda.init()
+    return torch

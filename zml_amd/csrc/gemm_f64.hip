@@ -1,0 +1,202 @@
+// FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4_f64), column-major view.
+//
+//   C = alpha * op(A) * op(B) + beta * C        (reference blas/gemm.zig:68-1033)
+//
+// Row-major calls are mapped here by swapping the operands and m/n, exactly
+// as zml does (gemm.zig:48-65). Only the arithmetic differs from zml: the
+// matrix cores accumulate with fused multiply-adds, so results agree with
+// zml's separately rounded loop to within the usual gamma_k |A||B| bound
+// (bitwise on integer-valued data such as the reference's golden vectors).
+//
+// Tiling: a 256-thread workgroup computes a 128 x 128 tile of C; its four
+// waves each own 64 x 64 (4 x 4 MFMA tiles of 16 x 16). K advances in slabs
+// of 16 staged through double-buffered LDS (both operands stored k-major,
+// so every MFMA operand fetch is a 16-lane contiguous read). The MFMA is
+// issued as C^T = op(B)^T op(A)^T so that the 16 lanes of an output group
+// hold 16 consecutive rows of one column of C: stores and beta-loads are
+// 128-byte contiguous in column-major memory.
+#include "zmlhip_internal.hpp"
+
+namespace zmlhip {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
+constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
+constexpr int NT = 256;
+
+// Maps the linear workgroup id to a (tile_m, tile_n) pair: consecutive ids
+// land on different XCDs (dispatch is round-robin over 8 XCDs), so the ids
+// are first regrouped so each XCD gets a contiguous range of tiles, and the
+// range is walked in groups of 8 tile-rows to share operand panels in L2.
+__device__ __forceinline__ void tile_coords(int64_t bid, int64_t ntm, int64_t ntn, int64_t& tm,
+                                            int64_t& tn) {
+  const int64_t nt = ntm * ntn;
+  int64_t pid = bid;
+  if ((nt & 7) == 0) pid = (bid & 7) * (nt >> 3) + (bid >> 3);
+  constexpr int64_t G = 8;
+  const int64_t per_group = G * ntn;
+  const int64_t group = pid / per_group;
+  const int64_t first_m = group * G;
+  const int64_t gm = (ntm - first_m) < G ? (ntm - first_m) : G;
+  const int64_t r = pid - group * per_group;
+  tm = first_m + r % gm;
+  tn = r / gm;
+}
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(NT, 2)
+gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __restrict__ A,
+                int64_t lda, const double* __restrict__ B, int64_t ldb, double beta,
+                double* __restrict__ C, int64_t ldc, int tri) {
+  __shared__ double As[2][BK * LDA_S];
+  __shared__ double Bs[2][BK * LDB_S];
+
+  const int64_t ntm = (m + BM - 1) / BM, ntn = (n + BN - 1) / BN;
+  int64_t tm, tn;
+  tile_coords(blockIdx.x, ntm, ntn, tm, tn);
+  const int64_t row0 = tm * BM, col0 = tn * BN;
+  if (tri == kLower && row0 + BM - 1 < col0) return;   // tile strictly above the diagonal
+  if (tri == kUpper && col0 + BN - 1 < row0) return;   // tile strictly below the diagonal
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // ---- global -> register staging (8 elements of A and 8 of B per thread)
+  double ra[8], rb[8];
+  auto load_tiles = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int64_t i, l;
+      if (!TA) { i = row0 + (t & 127); l = k0 + (t >> 7) + 2 * q; }   // m-contiguous
+      else     { l = k0 + (t & 15);    i = row0 + (t >> 4) + 16 * q; } // k-contiguous
+      const bool ok = i < m && l < k;
+      ra[q] = ok ? (TA ? A[l + i * lda] : A[i + l * lda]) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int64_t j, l;
+      if (TB) { j = col0 + (t & 127); l = k0 + (t >> 7) + 2 * q; }    // n-contiguous
+      else    { l = k0 + (t & 15);    j = col0 + (t >> 4) + 16 * q; }  // k-contiguous
+      const bool ok = j < n && l < k;
+      rb[q] = ok ? (TB ? B[j + l * ldb] : B[l + j * ldb]) : 0.0;
+    }
+  };
+  auto store_tiles = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int i, l;
+      if (!TA) { i = t & 127; l = (t >> 7) + 2 * q; }
+      else     { l = t & 15;  i = (t >> 4) + 16 * q; }
+      As[buf][l * LDA_S + i] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int j, l;
+      if (TB) { j = t & 127; l = (t >> 7) + 2 * q; }
+      else    { l = t & 15;  j = (t >> 4) + 16 * q; }
+      Bs[buf][l * LDB_S + j] = rb[q];
+    }
+  };
+
+  d4 acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int nk = (int)((k + BK - 1) / BK);
+  load_tiles(0);
+  store_tiles(0);
+  __syncthreads();
+
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load_tiles((int64_t)(kt + 1) * BK);
+    const double* as = As[cur] + wm * 64 + lr;
+    const double* bs = Bs[cur] + wn * 64 + lr;
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kr = 4 * ks + lk;
+      double av[4], bv[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) av[x] = as[kr * LDA_S + 16 * x];
+#pragma unroll
+      for (int y = 0; y < 4; ++y) bv[y] = bs[kr * LDB_S + 16 * y];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[y], av[x], acc[x][y], 0, 0, 0);
+    }
+    if (kt + 1 < nk) store_tiles(cur ^ 1);
+    __syncthreads();
+  }
+
+  // ---- epilogue: lane holds C[i0 + lr][j0 + lk + 4r] of each 16 x 16 tile
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const int64_t i = row0 + wm * 64 + 16 * x + lr;
+    if (i >= m) continue;
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t j = col0 + wn * 64 + 16 * y + lk + 4 * r;
+        if (j >= n) continue;
+        if (tri == kLower && i < j) continue;
+        if (tri == kUpper && i > j) continue;
+        double* cp = C + i + j * ldc;
+        const double v = alpha * acc[x][y][r];
+        *cp = beta == 0.0 ? v : (beta == 1.0 ? *cp + v : beta * *cp + v);
+      }
+    }
+  }
+}
+
+__global__ void scale_kernel(int64_t m, int64_t n, double beta, double* __restrict__ c,
+                             int64_t ldc, int tri) {
+  const int64_t total = m * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % m, j = e / m;
+    if (tri == kLower && i < j) continue;
+    if (tri == kUpper && i > j) continue;
+    double* p = c + i + j * ldc;
+    *p = beta == 0.0 ? 0.0 : *p * beta;
+  }
+}
+
+}  // namespace
+
+void scale_f64_cm(int64_t m, int64_t n, double beta, double* c, int64_t ldc, hipStream_t s,
+                  int tri) {
+  if (m <= 0 || n <= 0 || beta == 1.0) return;
+  int64_t total = m * n;
+  int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(scale_kernel, dim3(blocks), dim3(256), 0, s, m, n, beta, c, ldc, tri);
+  ZH_LAUNCH_CHECK();
+}
+
+void gemm_f64_cm(int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                 double* c, int64_t ldc, hipStream_t s, int tri) {
+  if (m <= 0 || n <= 0) return;
+  if (alpha == 0.0 || k <= 0) {  // gemm.zig:104-140: only the beta scaling remains
+    scale_f64_cm(m, n, beta, c, ldc, s, tri);
+    return;
+  }
+  const bool ta = !is_notrans(transa), tb = !is_notrans(transb);
+  const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+  dim3 grid((unsigned)tiles), block(NT);
+  if (!ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri);
+  else if (ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri);
+  else if (!ta && tb) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri);
+  else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri);
+  ZH_LAUNCH_CHECK();
+}
+
+}  // namespace zmlhip

@@ -1,0 +1,140 @@
+// Internal definitions shared by the HIP sources of libzmlhip.
+//
+// Conventions
+//  * All device entry points take a hipStream_t and are asynchronous; the
+//    host-pointer CBLAS/LAPACKE entry points (capi.cpp) are synchronous.
+//  * Enumerations carry the CBLAS/LAPACKE numeric values zml maps to
+//    (reference include/cblas.h:11-37).
+//  * "cm" in a name means the routine works on the column-major view; a
+//    row-major call is mapped onto it the way zml maps it (swap operands /
+//    invert side and uplo), see each routine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace zmlhip {
+
+enum Layout : int { kRowMajor = 101, kColMajor = 102 };
+enum Trans : int { kNoTrans = 111, kTrans = 112, kConjTrans = 113, kConjNoTrans = 114 };
+enum Uplo : int { kUpper = 121, kLower = 122 };
+enum Diag : int { kNonUnit = 131, kUnit = 132 };
+enum Side : int { kLeft = 141, kRight = 142 };
+
+// Status codes returned by the device entry points (0 = success). LAPACK
+// info values (> 0) and -i argument errors are returned as LAPACKE does.
+enum Status : int {
+  kOk = 0,
+  kErrHip = -1000,       // a HIP runtime call failed
+  kErrAlloc = -1001,     // device allocation failed
+  kErrSync = -1002,      // a bounded device-side wait timed out
+  kErrNotSupported = -1003,
+};
+
+inline bool is_notrans(int t) { return t == kNoTrans || t == kConjNoTrans; }
+inline bool valid_trans(int t) { return t >= kNoTrans && t <= kConjNoTrans; }
+inline bool valid_layout(int l) { return l == kRowMajor || l == kColMajor; }
+inline bool valid_uplo(int u) { return u == kUpper || u == kLower; }
+inline bool valid_diag(int d) { return d == kNonUnit || d == kUnit; }
+inline bool valid_side(int s) { return s == kLeft || s == kRight; }
+inline int invert_uplo(int u) { return u == kUpper ? kLower : kUpper; }
+inline int invert_side(int s) { return s == kLeft ? kRight : kLeft; }
+inline int invert_trans(int t) {
+  switch (t) {
+    case kNoTrans: return kTrans;
+    case kTrans: return kNoTrans;
+    case kConjTrans: return kConjNoTrans;
+    default: return kConjTrans;
+  }
+}
+
+struct HipError : std::runtime_error {
+  hipError_t code;
+  HipError(hipError_t c, const char* what) : std::runtime_error(what), code(c) {}
+};
+
+#define ZH_CHECK(expr)                                                                      \
+  do {                                                                                      \
+    hipError_t zh_err_ = (expr);                                                            \
+    if (zh_err_ != hipSuccess) {                                                            \
+      char zh_buf_[512];                                                                    \
+      snprintf(zh_buf_, sizeof(zh_buf_), "%s failed: %s (%s:%d)", #expr,                    \
+               hipGetErrorString(zh_err_), __FILE__, __LINE__);                             \
+      throw ::zmlhip::HipError(zh_err_, zh_buf_);                                           \
+    }                                                                                       \
+  } while (0)
+
+#define ZH_LAUNCH_CHECK() ZH_CHECK(hipGetLastError())
+
+// Last error message (per thread), readable through zmlhip_last_error().
+void set_last_error(const std::string& msg);
+const char* last_error();
+
+// Growable device workspace owned by the library, one per (thread, device).
+// Handed-out pointers stay valid until the next ensure() with a larger size.
+void* workspace(size_t bytes, int slot = 0);
+
+// Stream used by the synchronous host-pointer entry points of this thread.
+hipStream_t host_api_stream();
+
+// Number of compute units of the current device.
+int device_cu_count();
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ---- kernels exposed between translation units -----------------------------
+
+// Column-major FP64 GEMM: C = alpha * op(A) * op(B) + beta * C.
+// tri: 0 full, kLower / kUpper restrict the update to that triangle of C
+// (tiles outside it are skipped, diagonal tiles are masked), used by SYRK.
+void gemm_f64_cm(int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                 double* c, int64_t ldc, hipStream_t s, int tri = 0);
+
+// Column-major triangular solve (zml trsm semantics, column-major view).
+void trsm_f64_cm(int side, int uplo, int transa, int diag, int64_t m, int64_t n, double alpha,
+                 const double* a, int64_t lda, double* b, int64_t ldb, hipStream_t s);
+
+// Column-major SYRK: C = alpha * op(A) op(A)^T + beta * C on one triangle.
+void syrk_f64_cm(int uplo, int trans, int64_t n, int64_t k, double alpha, const double* a,
+                 int64_t lda, double beta, double* c, int64_t ldc, hipStream_t s);
+
+// Scale an m x n column-major block: c = beta * c (beta == 0 writes zeros).
+void scale_f64_cm(int64_t m, int64_t n, double beta, double* c, int64_t ldc, hipStream_t s,
+                  int tri = 0);
+
+// Row interchanges on a row-major matrix: for i = k1, k1+inc, ..., k2 (0-based,
+// inclusive, inc = +1/-1) swap rows i and ipiv[ix]-1 (ipiv 1-based) over
+// columns [0, ncols); ix starts at ix0 and advances by incx (default: ix = i).
+void laswp_rm(int64_t ncols, double* a, int64_t lda, int k1, int k2, const int32_t* ipiv,
+              int inc, hipStream_t s, int ix0 = -1, int incx = 0);
+// Same for a column-major matrix (rows are strided).
+void laswp_cm(int64_t ncols, double* a, int64_t lda, int k1, int k2, const int32_t* ipiv,
+              int inc, hipStream_t s, int ix0 = -1, int incx = 0);
+
+// Level-1 and Level-2 helpers (blas_f64.hip).
+void iamax_f64(int64_t n, const double* x, int64_t incx, int32_t* out, hipStream_t s);
+void scal_f64(int64_t n, double alpha, double* x, int64_t incx, hipStream_t s);
+void swap_f64(int64_t n, double* x, int64_t incx, double* y, int64_t incy, hipStream_t s);
+void gemv_f64_cm(int trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
+                 const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s);
+
+// LAPACK drivers (getrf_f64.hip, potrf_f64.hip).
+void getrf_f64_rm(int64_t m, int64_t n, double* a, int64_t lda, int32_t* ipiv, int* info,
+                  hipStream_t s);
+int leaf_error_flag(hipStream_t s);
+void potrf_f64_cm(int uplo, int64_t n, double* a, int64_t lda, int* info, hipStream_t s);
+
+// In-place transpose of the leading n x n block of a matrix with leading
+// dimension ld (switches between row- and column-major storage).
+void transpose_square_inplace(int64_t n, double* a, int64_t ld, hipStream_t s);
+// Out-of-place: dst (cols x rows, ldd) = src^T (src rows x cols, lds; row-major view).
+void transpose_copy(int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst,
+                    int64_t ldd, hipStream_t s);
+
+}  // namespace zmlhip
