@@ -1,0 +1,389 @@
+// LU panel machinery shared by the FP64 and the mixed-precision drivers:
+//   * the leaf kernel: partial-pivoting LU of a tall M x W row-major panel,
+//     one row per thread held in registers, all workgroups resident;
+//   * row-interchange application through the composed permutation.
+//
+// Cross-workgroup exchange (leaf kernel, more than one workgroup). Per
+// column k every workgroup publishes, into slot k mod 2,
+//   val[g]  = |a| of its candidate (largest |a_ik| among its rows)
+//   rec[g]  = {row index, candidate row}
+//   rowk    = {NaN flag, row k}              (owner of row k only)
+// Every 64-bit word carries a 32-bit tag (launch base + k + 1) above 32
+// payload bits (a double is split over two words), stored and loaded as
+// device-scope relaxed atomics, which the memory model keeps coherent
+// across XCDs without cache-wide fences. A reader knows a word is current
+// from its tag alone, so writers never wait for their stores: a reader polls
+// the G val words, picks the winner (largest |a|, lowest workgroup on ties =
+// lowest row, since workgroups own increasing row ranges: iamax.zig:32's
+// strict '>'), then fetches the winner's record and row k, re-reading any
+// word whose tag is not yet current. A slot is rewritten two columns later,
+// when every workgroup has provably finished reading it (it has published
+// the column in between).
+#pragma once
+
+#include "zmlhip_internal.hpp"
+
+namespace zmlhip {
+namespace panel {
+
+constexpr int W = 64;            // leaf width (columns held in registers)
+// threads (= rows) per workgroup: FP64 rows take 128 VGPRs, so one wave per
+// SIMD keeps the whole register file (incl. AGPRs) available to a thread.
+template <typename T> constexpr int pt_of() { return sizeof(T) == 8 ? 256 : 512; }
+constexpr int GMAX = 256;        // workgroups per leaf
+constexpr int REC = 2 + 2 * W;   // words per record: index/flag + 2 per value (+ pad)
+
+struct WS {
+  uint64_t* val;    // [2][GMAX][2]
+  uint64_t* rec;    // [2][GMAX][REC]
+  uint64_t* rowk;   // [2][REC]
+  int* err;
+};
+
+inline size_t ws_bytes() {
+  return sizeof(uint64_t) * (2 * GMAX * 2 + 2 * GMAX * REC + 2 * REC) + 256;
+}
+inline WS carve(void* base) {
+  WS w;
+  uint64_t* p = static_cast<uint64_t*>(base);
+  w.val = p; p += 2 * GMAX * 2;
+  w.rec = p; p += 2 * GMAX * REC;
+  w.rowk = p; p += 2 * REC;
+  w.err = reinterpret_cast<int*>(p);
+  return w;
+}
+
+__device__ __forceinline__ void st64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld64(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t tagged(uint32_t tag, uint32_t payload) {
+  return ((uint64_t)tag << 32) | payload;
+}
+__device__ __forceinline__ uint32_t tag_of(uint64_t w) { return (uint32_t)(w >> 32); }
+__device__ __forceinline__ uint32_t payload(uint64_t w) { return (uint32_t)w; }
+// a value travels as a double split over two tagged words
+template <typename T>
+__device__ __forceinline__ void st_val(uint64_t* p, uint32_t tag, T v) {
+  const uint64_t b = (uint64_t)__double_as_longlong((double)v);
+  st64(p, tagged(tag, (uint32_t)(b >> 32)));
+  st64(p + 1, tagged(tag, (uint32_t)b));
+}
+template <typename T>
+__device__ __forceinline__ T val_of(uint64_t hi, uint64_t lo) {
+  return (T)__longlong_as_double((long long)(((uint64_t)payload(hi) << 32) | payload(lo)));
+}
+
+// (value, index) maximum with the lowest index winning ties.
+template <typename V, typename I>
+__device__ __forceinline__ void amax(V& v, I& i, V v2, I i2) {
+  if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
+}
+
+// Rank-1 update of one row for column k: a[k] <- l, a[c] <- a[c] - l u[c]
+// for c > k, separately rounded. The register row is handled in groups of 8
+// columns so that every index is a compile-time constant; k is uniform, so
+// the group tests are scalar branches and only the group holding k needs
+// per-column predication.
+template <int G8, typename T>
+__device__ __forceinline__ void update_group(T (&a)[W], T l, const T* __restrict__ u, int k) {
+#pragma clang fp contract(off)
+  constexpr int c0 = 8 * G8;
+  if (c0 > k) {
+#pragma unroll
+    for (int c = c0; c < c0 + 8; ++c) a[c] = a[c] - l * u[c];
+  } else if (c0 + 7 >= k) {
+#pragma unroll
+    for (int c = c0; c < c0 + 8; ++c) {
+      if (c == k) a[c] = l;
+      else if (c > k) a[c] = a[c] - l * u[c];
+    }
+  }
+}
+
+template <int G8, typename T>
+__device__ __forceinline__ void pick_group(const T (&a)[W], int c, T& out) {
+  constexpr int c0 = 8 * G8;
+  if (c >= c0 && c < c0 + 8) {
+#pragma unroll
+    for (int x = c0; x < c0 + 8; ++x) if (x == c) out = a[x];
+  }
+}
+
+// a[c] for a uniform, run-time c (0 if c >= W).
+template <typename T>
+__device__ __forceinline__ T pick(const T (&a)[W], int c) {
+  T out = T(0);
+  pick_group<0>(a, c, out); pick_group<1>(a, c, out); pick_group<2>(a, c, out); pick_group<3>(a, c, out);
+  pick_group<4>(a, c, out); pick_group<5>(a, c, out); pick_group<6>(a, c, out); pick_group<7>(a, c, out);
+  return out;
+}
+
+template <typename T>
+__device__ __forceinline__ void update_row(T (&a)[W], T l, const T* __restrict__ u, int k) {
+  update_group<0>(a, l, u, k); update_group<1>(a, l, u, k); update_group<2>(a, l, u, k);
+  update_group<3>(a, l, u, k); update_group<4>(a, l, u, k); update_group<5>(a, l, u, k);
+  update_group<6>(a, l, u, k); update_group<7>(a, l, u, k);
+}
+
+// Leaf LU: rows [0, M) x columns [0, Wn) of the row-major panel P (Wn <= 64).
+// ipiv[k] = piv_base + pivot row + 1 (1-based, global); *info gets
+// info_base + first zero pivot (1-based) when it is still 0.
+// MULTI: more than one workgroup (exchange through ws), else LDS only.
+template <typename T, bool MULTI>
+__global__ void __launch_bounds__(pt_of<T>())
+leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restrict__ ipiv,
+            int64_t piv_base, int* __restrict__ info, int64_t info_base, WS ws, uint32_t tag_base,
+            long spin_limit) {
+#pragma clang fp contract(off)
+  constexpr int PT = pt_of<T>();
+  const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  constexpr int NW = PT / 64;
+  const int64_t i = (int64_t)g * PT + t;
+  const bool own = i < M;
+
+  __shared__ T u[W], rk[W];
+  __shared__ T s_v[NW];
+  __shared__ int64_t s_i[NW];
+  __shared__ int64_t s_p;
+  __shared__ int s_nan, s_abort;
+  __shared__ int32_t s_piv[W];
+
+  T a[W];
+#pragma unroll
+  for (int c = 0; c < W; ++c) a[c] = (own && c < Wn) ? P[i * ld + c] : T(0);
+  if (t == 0) { s_abort = 0; s_nan = 0; }
+
+  int first_zero = 0;
+  const T sfmin = sizeof(T) == 8 ? T(0x1p-1022) : T(0x1p-126);   // lamch('S')
+  T ak = a[0];
+
+  for (int k = 0; k < Wn; ++k) {
+    // ---- workgroup candidate: largest |a_ik| over own rows i >= k (NaN never
+    // wins). The wave maximum is found on the value alone, the lowest lane
+    // holding it by a ballot (lanes hold increasing rows).
+    T v = T(-1);
+    if (own && i >= k) {
+      const T f = fabs(ak);
+      if (f == f) v = f;
+    }
+    T wmax = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
+    const uint64_t hit = __ballot(v == wmax && v >= T(0));
+    if (lane == 0) {
+      s_v[wv] = wmax;
+      s_i[wv] = hit ? (int64_t)g * PT + wv * 64 + (__ffsll((unsigned long long)hit) - 1) : INT64_MAX;
+    }
+    __syncthreads();
+    T bv = s_v[0];
+    int64_t bi = s_i[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) amax(bv, bi, s_v[w], s_i[w]);
+
+    if (!MULTI) {
+      if (own && i == bi) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) u[c] = a[c];
+        s_p = i;
+      }
+      if (own && i == k) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) rk[c] = a[c];
+        s_nan = (ak != ak);
+      }
+      __syncthreads();
+      if (t == 0 && (s_nan || bi == INT64_MAX)) s_p = k;
+      if (s_nan && t < W) u[t] = rk[t];
+      __syncthreads();
+    } else {
+      const int slot = k & 1;
+      const uint32_t tag = tag_base + (uint32_t)k + 1u;
+      uint64_t* my_rec = ws.rec + ((size_t)slot * GMAX + g) * REC;
+      uint64_t* rkrec = ws.rowk + (size_t)slot * REC;
+      // publish: candidate row and row k straight from their owners' registers
+      if (own && i == bi) {
+        st64(my_rec, tagged(tag, (uint32_t)i));
+#pragma unroll
+        for (int c = 0; c < W; ++c) st_val<T>(my_rec + 2 + 2 * c, tag, a[c]);
+      }
+      if (own && i == k) {
+        st64(rkrec, tagged(tag, (ak != ak) ? 1u : 0u));
+#pragma unroll
+        for (int c = 0; c < W; ++c) st_val<T>(rkrec + 2 + 2 * c, tag, a[c]);
+      }
+      if (t == 0) st_val<T>(ws.val + ((size_t)slot * GMAX + g) * 2, tag, bi == INT64_MAX ? T(-1) : bv);
+      if (wv == 0) {
+        long spins = 0;
+        // (1) every workgroup's |a|
+        T gv = T(-1);
+        int gg = INT32_MAX;
+        while (true) {
+          bool ok = true;
+          gv = T(-1);
+          gg = INT32_MAX;
+          for (int w = lane; w < G; w += 64) {
+            const uint64_t* q = ws.val + ((size_t)slot * GMAX + w) * 2;
+            const uint64_t hi = ld64(q), lo = ld64(q + 1);
+            if (tag_of(hi) != tag || tag_of(lo) != tag) ok = false;
+            else amax(gv, gg, val_of<T>(hi, lo), w);
+          }
+          if (__all(ok)) break;
+          if (++spins > spin_limit) { if (lane == 0) { s_abort = 1; atomicExch(ws.err, 1); } break; }
+          __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const T v2 = __shfl_xor(gv, off);
+          const int g2 = __shfl_xor(gg, off);
+          amax(gv, gg, v2, g2);
+        }
+        // (2) the winner's record and row k
+        if (!s_abort) {
+          const uint64_t* r = ws.rec + ((size_t)slot * GMAX + (gg == INT32_MAX ? 0 : gg)) * REC;
+          const bool want_r = gv >= T(0);
+          while (true) {
+            const uint64_t rh = ld64(r + 2 + 2 * lane), rl = ld64(r + 3 + 2 * lane);
+            const uint64_t kh = ld64(rkrec + 2 + 2 * lane), kl = ld64(rkrec + 3 + 2 * lane);
+            const uint64_t pw = ld64(r), nw = ld64(rkrec);
+            const bool ok = (!want_r || (tag_of(rh) == tag && tag_of(rl) == tag && tag_of(pw) == tag)) &&
+                            tag_of(kh) == tag && tag_of(kl) == tag && tag_of(nw) == tag;
+            if (__all(ok)) {
+              const bool nanf = payload(nw) != 0;
+              rk[lane] = val_of<T>(kh, kl);
+              u[lane] = (nanf || !want_r) ? val_of<T>(kh, kl) : val_of<T>(rh, rl);
+              if (lane == 0) s_p = (nanf || !want_r) ? (int64_t)k : (int64_t)payload(pw);
+              break;
+            }
+            if (++spins > spin_limit) { if (lane == 0) { s_abort = 1; atomicExch(ws.err, 1); } break; }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+      }
+      __syncthreads();
+      if (s_abort) return;
+    }
+
+    const int64_t p = s_p;
+    const T piv = u[k];
+    const bool nz = piv != T(0);   // getrf2.zig:55: no interchange on an exact zero
+    if (!nz && first_zero == 0) first_zero = k + 1;
+    // pivots are written out at the end: a global store here would make the
+    // next barrier wait for its completion
+    if (t == 0) s_piv[k] = (int32_t)(piv_base + (nz ? p : k) + 1);
+    T l = ak;
+    if (nz && p != k) {
+      if (own && i == k) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) a[c] = u[c];
+      } else if (own && i == p) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) a[c] = rk[c];
+        l = rk[k];
+      }
+    }
+    if (own && i > k) {
+      if (nz) {
+        if (fabs(piv) >= sfmin) {
+          const T r = T(1) / piv;
+          l = l * r;
+        } else {
+          l = l / piv;
+        }
+      }
+      update_row<T>(a, l, u, k);
+    }
+    ak = pick<T>(a, k + 1);   // candidate value of the next column
+  }
+
+#pragma unroll
+  for (int c = 0; c < W; ++c)
+    if (own && c < Wn) P[i * ld + c] = a[c];
+  if (g == 0 && t < Wn) ipiv[t] = s_piv[t];
+  if (g == 0 && t == 0 && first_zero != 0 && *info == 0) *info = (int)(info_base + first_zero);
+}
+
+// ---- row interchanges through the composed permutation ----------------------
+constexpr int PERM_MAX = 512;   // pairs (2 x block of 256 interchanges)
+
+// Composes the interchanges i <-> ipiv[i]-1 for i = k1..k2 (ascending, every
+// ipiv[i]-1 >= i) into a list of moves dst <- src over the rows touched.
+// One wave. pairs[0] = count, then (dst, src) pairs.
+__global__ void compose_perm_kernel(int k1, int k2, const int32_t* __restrict__ ipiv,
+                                    int32_t* __restrict__ pairs) {
+  __shared__ int32_t pos[PERM_MAX], cont[PERM_MAX];
+  __shared__ int s_n;
+  const int lane = threadIdx.x;
+  const int kb = k2 - k1 + 1;
+  for (int q = lane; q < kb; q += 64) { pos[q] = k1 + q; cont[q] = k1 + q; }
+  if (lane == 0) s_n = kb;
+  __syncthreads();
+  for (int i = k1; i <= k2; ++i) {
+    const int p = ipiv[i] - 1;
+    int qp;
+    if (p <= k2) {
+      qp = p - k1;
+    } else {
+      const int n = s_n;
+      int found = INT32_MAX;
+      for (int q = kb + lane; q < n; q += 64)
+        if (pos[q] == p) found = q;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) found = min(found, __shfl_xor(found, off));
+      if (found == INT32_MAX) {
+        found = n;
+        if (lane == 0) { pos[n] = p; cont[n] = p; s_n = n + 1; }
+      }
+      qp = found;
+    }
+    __syncthreads();
+    if (lane == 0 && qp != i - k1) {
+      const int tmp = cont[i - k1];
+      cont[i - k1] = cont[qp];
+      cont[qp] = tmp;
+    }
+    __syncthreads();
+  }
+  const int n = s_n;
+  if (lane == 0) {
+    int c = 0;
+    for (int q = 0; q < n; ++q)
+      if (cont[q] != pos[q]) {
+        pairs[1 + 2 * c] = pos[q];
+        pairs[2 + 2 * c] = cont[q];
+        ++c;
+      }
+    pairs[0] = c;
+  }
+}
+
+// Applies the moves to columns [0, ncols) of a row-major matrix: every
+// workgroup owns 16 columns, reads all source rows, then writes them.
+template <typename T>
+__global__ void __launch_bounds__(256)
+apply_perm_kernel(int64_t ncols, T* __restrict__ a, int64_t lda, const int32_t* __restrict__ pairs) {
+  constexpr int CW = 16, RPT = PERM_MAX / (256 / CW);   // 32 moves per thread
+  const int n = pairs[0];
+  const int col = threadIdx.x % CW, q0 = threadIdx.x / CW;
+  const int64_t c = blockIdx.x * (int64_t)CW + col;
+  const bool in = c < ncols;
+  T v[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int q = q0 + r * (256 / CW);
+    if (in && q < n) v[r] = a[(int64_t)pairs[2 + 2 * q] * lda + c];
+  }
+  // a column chunk belongs to one workgroup: all of its reads precede all of its writes
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int q = q0 + r * (256 / CW);
+    if (in && q < n) a[(int64_t)pairs[1 + 2 * q] * lda + c] = v[r];
+  }
+}
+
+}  // namespace panel
+}  // namespace zmlhip
