@@ -33,8 +33,10 @@ constexpr int NT = 256;
 __device__ __forceinline__ void tile_coords(int64_t bid, int64_t ntm, int64_t ntn, int64_t& tm,
                                             int64_t& tn) {
   const int64_t nt = ntm * ntn;
-  int64_t pid = bid;
-  if ((nt & 7) == 0) pid = (bid & 7) * (nt >> 3) + (bid >> 3);
+  // XCD x receives the ids bid = x, x + 8, ...: give it the contiguous range
+  // [x * q + min(x, r), ...) of tiles, q = nt / 8, r = nt % 8
+  const int64_t x = bid & 7, q = nt >> 3, r8 = nt & 7;
+  const int64_t pid = x * q + (x < r8 ? x : r8) + (bid >> 3);
   constexpr int64_t G = 8;
   const int64_t per_group = G * ntn;
   const int64_t group = pid / per_group;

@@ -42,6 +42,23 @@ static void trsm_rm_llnu(int64_t m, int64_t n, const double* a, int64_t lda, dou
   trsm_f64_cm(kRight, kUpper, kNoTrans, kUnit, n, m, 1.0, a, lda, b, ldb, s);
 }
 
+// Trailing-update kernels of a row-major LU: C -= L U and B <- L11^-1 B.
+// The wide U12 solve goes through the inverse of the unit lower L11: the
+// panel stream forms L11^-1 right after factoring the block column, and the
+// main stream then computes U12 = L11^-1 A12 as one GEMM on a copy of A12.
+template <typename T>
+struct LuOps {
+  void (*gemm)(int64_t rows, int64_t cols, int64_t kb, const T* L, int64_t ldl, const T* U,
+               int64_t ldu, T* C, int64_t ldc, hipStream_t s);
+  void (*trsm)(int64_t kb, int64_t cols, const T* L11, int64_t ld, T* B, int64_t ldb,
+               hipStream_t s);
+  // inv (kb x kb, ld kb) <- L11^-1
+  void (*invert)(int64_t kb, const T* L11, int64_t ld, T* inv, hipStream_t s);
+  // B (kb x cols) <- inv * B, through tmp (kb x cols, ld cols)
+  void (*apply_inv)(int64_t kb, int64_t cols, const T* inv, T* B, int64_t ldb, T* tmp,
+                    hipStream_t s);
+};
+
 struct LeafState {
   void* ws = nullptr;
   int launches = 0;
@@ -97,10 +114,11 @@ template void leaf_factor<double>(int64_t, double*, int64_t, int64_t, int64_t, i
 template void leaf_factor<float>(int64_t, float*, int64_t, int64_t, int64_t, int, int32_t*, int*, hipStream_t);
 
 // Interchanges ipiv[k1..k2] (ascending) applied to `ncols` columns starting
-// at `a` of a row-major matrix, through the composed permutation.
-int32_t* compose_perm(int k1, int k2, const int32_t* ipiv, hipStream_t s) {
+// at `a` of a row-major matrix, through the composed permutation. Each
+// stream that composes permutations uses its own workspace slot.
+int32_t* compose_perm(int k1, int k2, const int32_t* ipiv, hipStream_t s, int slot) {
   if (k2 - k1 + 1 > panel::PERM_MAX / 2) throw std::runtime_error("compose_perm: block too wide");
-  int32_t* pairs = static_cast<int32_t*>(workspace(sizeof(int32_t) * (2 * panel::PERM_MAX + 4), 4));
+  int32_t* pairs = static_cast<int32_t*>(workspace(sizeof(int32_t) * (2 * panel::PERM_MAX + 4), slot));
   hipLaunchKernelGGL(panel::compose_perm_kernel, dim3(1), dim3(64), 0, s, k1, k2, ipiv, pairs);
   ZH_LAUNCH_CHECK();
   return pairs;
@@ -115,45 +133,109 @@ void apply_perm(int64_t ncols, T* a, int64_t lda, const int32_t* pairs, hipStrea
 template void apply_perm<double>(int64_t, double*, int64_t, const int32_t*, hipStream_t);
 template void apply_perm<float>(int64_t, float*, int64_t, const int32_t*, hipStream_t);
 
-// Row-major FP64 getrf on device memory. ipiv (1-based, min(m,n) entries) and
-// info live in device memory; info must be zero on entry.
-void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, int* info,
-                  hipStream_t s) {
-  const int64_t mn = std::min(m, n);
-  for (int64_t j0 = 0; j0 < mn; j0 += NB_F64) {
-    const int64_t jb = std::min<int64_t>(NB_F64, mn - j0);
-    // ---- panel, in leaves
-    for (int64_t k0 = j0; k0 < j0 + jb; k0 += panel::W) {
-      const int kb = (int)std::min<int64_t>(panel::W, j0 + jb - k0);
-      leaf_factor<double>(m, A, lda, k0, k0, kb, ipiv, info, s);
-      // interchanges of this leaf on the other panel columns
-      const int64_t rc = j0 + jb - (k0 + kb);
-      if (k0 > j0 || rc > 0) {
-        const int32_t* pr = compose_perm((int)k0, (int)(k0 + kb - 1), ipiv, s);
-        apply_perm<double>(k0 - j0, A + j0, lda, pr, s);
-        apply_perm<double>(rc, A + k0 + kb, lda, pr, s);
-      }
-      if (rc > 0) {
-        trsm_rm_llnu(kb, rc, A + k0 * lda + k0, lda, A + k0 * lda + k0 + kb, lda, s);
-        if (m - k0 - kb > 0)
-          gemm_rm_nn(m - k0 - kb, rc, kb, -1.0, A + (k0 + kb) * lda + k0, lda, A + k0 * lda + k0 + kb,
-                     lda, 1.0, A + (k0 + kb) * lda + k0 + kb, lda, s);
-      }
+// Factor the block column [j0, j0 + jb) (rows j0..m-1) of a row-major matrix
+// in leaves; interchanges are applied inside the block column only.
+template <typename T>
+void factor_panel(int64_t m, T* A, int64_t lda, int64_t j0, int64_t jb, int32_t* ipiv, int* info,
+                  const LuOps<T>& ops, hipStream_t s, int perm_slot) {
+  for (int64_t k0 = j0; k0 < j0 + jb; k0 += panel::W) {
+    const int kb = (int)std::min<int64_t>(panel::W, j0 + jb - k0);
+    leaf_factor<T>(m, A, lda, k0, k0, kb, ipiv, info, s);
+    const int64_t rc = j0 + jb - (k0 + kb);
+    if (k0 > j0 || rc > 0) {
+      const int32_t* pr = compose_perm((int)k0, (int)(k0 + kb - 1), ipiv, s, perm_slot);
+      apply_perm<T>(k0 - j0, A + j0, lda, pr, s);
+      apply_perm<T>(rc, A + k0 + kb, lda, pr, s);
     }
-    // ---- interchanges outside the panel (getrf.zig:82-104)
-    const int64_t nr = n - j0 - jb;
-    if (j0 > 0 || nr > 0) {
-      const int32_t* pr = compose_perm((int)j0, (int)(j0 + jb - 1), ipiv, s);
-      apply_perm<double>(j0, A, lda, pr, s);
-      apply_perm<double>(nr, A + j0 + jb, lda, pr, s);
-    }
-    if (nr > 0) {
-      trsm_rm_llnu(jb, nr, A + j0 * lda + j0, lda, A + j0 * lda + j0 + jb, lda, s);
-      if (m - j0 - jb > 0)
-        gemm_rm_nn(m - j0 - jb, nr, jb, -1.0, A + (j0 + jb) * lda + j0, lda, A + j0 * lda + j0 + jb,
-                   lda, 1.0, A + (j0 + jb) * lda + j0 + jb, lda, s);
+    if (rc > 0) {
+      ops.trsm(kb, rc, A + k0 * lda + k0, lda, A + k0 * lda + k0 + kb, lda, s);
+      if (m - k0 - kb > 0)
+        ops.gemm(m - k0 - kb, rc, kb, A + (k0 + kb) * lda + k0, lda, A + k0 * lda + k0 + kb, lda,
+                 A + (k0 + kb) * lda + k0 + kb, lda, s);
     }
   }
+}
+
+// Row-major right-looking LU with look-ahead of depth one: while the main
+// stream applies step b to the columns beyond block b+1, the panel stream
+// already factors block b+1 (whose columns step b updated first). ipiv
+// (1-based) and info live in device memory; info must be zero on entry.
+template <typename T>
+void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info, int64_t nb,
+                     const LuOps<T>& ops, hipStream_t s) {
+  const int64_t mn = std::min(m, n);
+  if (mn <= 0) return;
+  hipStream_t sp = aux_stream(0);
+  hipEvent_t ev_in = aux_event(0), ev_panel = aux_event(1), ev_next = aux_event(2);
+  // L11^-1 of consecutive block columns alternate between two buffers; the
+  // copy of A12 is used by the main stream only
+  T* inv = static_cast<T*>(workspace(sizeof(T) * 2 * nb * nb, 0));
+  T* tmp = static_cast<T*>(workspace(sizeof(T) * nb * (size_t)std::max<int64_t>(n, 1), 1));
+  auto panel_and_inverse = [&](int64_t j0, int64_t jb, int64_t blk) {
+    factor_panel<T>(m, A, lda, j0, jb, ipiv, info, ops, sp, 2);
+    if (j0 + jb < n) ops.invert(jb, A + j0 * lda + j0, lda, inv + (blk & 1) * nb * nb, sp);
+    ZH_CHECK(hipEventRecord(ev_panel, sp));
+  };
+  ZH_CHECK(hipEventRecord(ev_in, s));
+  ZH_CHECK(hipStreamWaitEvent(sp, ev_in, 0));
+  panel_and_inverse(0, std::min(nb, mn), 0);
+  for (int64_t j0 = 0, blk = 0; j0 < mn; j0 += nb, ++blk) {
+    const int64_t jb = std::min(nb, mn - j0);
+    const T* invb = inv + (blk & 1) * nb * nb;
+    ZH_CHECK(hipStreamWaitEvent(s, ev_panel, 0));
+    const int32_t* pr = compose_perm((int)j0, (int)(j0 + jb - 1), ipiv, s, 4);
+    const int64_t c1 = j0 + jb;                       // first column right of the block
+    const int64_t jb2 = std::min(nb, std::max<int64_t>(0, mn - c1));   // next block width
+    const int64_t rows = m - c1;
+    if (jb2 > 0) {
+      // update the next block column first, then hand it to the panel stream
+      apply_perm<T>(jb2, A + c1, lda, pr, s);
+      ops.apply_inv(jb, jb2, invb, A + j0 * lda + c1, lda, tmp, s);
+      if (rows > 0) ops.gemm(rows, jb2, jb, A + c1 * lda + j0, lda, A + j0 * lda + c1, lda, A + c1 * lda + c1, lda, s);
+      ZH_CHECK(hipEventRecord(ev_next, s));
+      ZH_CHECK(hipStreamWaitEvent(sp, ev_next, 0));
+      panel_and_inverse(c1, jb2, blk + 1);
+    }
+    // the rest: interchanges left of the block and beyond the next one, then update
+    apply_perm<T>(j0, A, lda, pr, s);
+    const int64_t c2 = c1 + jb2, nr = n - c2;
+    if (nr > 0) {
+      apply_perm<T>(nr, A + c2, lda, pr, s);
+      ops.apply_inv(jb, nr, invb, A + j0 * lda + c2, lda, tmp, s);
+      if (rows > 0) ops.gemm(rows, nr, jb, A + c1 * lda + j0, lda, A + j0 * lda + c2, lda, A + c1 * lda + c2, lda, s);
+    }
+  }
+}
+
+static void gemm_f64_update(int64_t rows, int64_t cols, int64_t kb, const double* L, int64_t ldl,
+                            const double* U, int64_t ldu, double* C, int64_t ldc, hipStream_t s) {
+  gemm_rm_nn(rows, cols, kb, -1.0, L, ldl, U, ldu, 1.0, C, ldc, s);
+}
+static void trsm_f64_update(int64_t kb, int64_t cols, const double* L11, int64_t ld, double* B,
+                            int64_t ldb, hipStream_t s) {
+  trsm_rm_llnu(kb, cols, L11, ld, B, ldb, s);
+}
+
+__global__ void set_identity_kernel(int64_t n, double* __restrict__ a) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e < n * n) a[e] = (e / n == e % n) ? 1.0 : 0.0;
+}
+static void invert_f64(int64_t kb, const double* L11, int64_t ld, double* inv, hipStream_t s) {
+  hipLaunchKernelGGL(set_identity_kernel, dim3((unsigned)((kb * kb + 255) / 256)), dim3(256), 0, s, kb, inv);
+  ZH_LAUNCH_CHECK();
+  trsm_rm_llnu(kb, kb, L11, ld, inv, kb, s);
+}
+static void apply_inv_f64(int64_t kb, int64_t cols, const double* inv, double* B, int64_t ldb,
+                          double* tmp, hipStream_t s) {
+  ZH_CHECK(hipMemcpy2DAsync(tmp, sizeof(double) * cols, B, sizeof(double) * ldb, sizeof(double) * cols,
+                            kb, hipMemcpyDeviceToDevice, s));
+  gemm_rm_nn(kb, cols, kb, 1.0, inv, kb, tmp, cols, 0.0, B, ldb, s);
+}
+
+void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, int* info,
+                  hipStream_t s) {
+  const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64};
+  getrf_lookahead<double>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }
 
 int leaf_error_flag(hipStream_t s) {

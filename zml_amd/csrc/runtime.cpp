@@ -16,9 +16,12 @@ struct Buf {
   size_t bytes = 0;
 };
 constexpr int kSlots = 8;
+constexpr int kAux = 4, kEvents = 8;
 struct DeviceState {
   Buf slot[kSlots];
   hipStream_t stream = nullptr;
+  hipStream_t aux[kAux] = {};
+  hipEvent_t ev[kEvents] = {};
 };
 struct ThreadState {
   std::unordered_map<int, DeviceState> dev;
@@ -30,6 +33,10 @@ struct ThreadState {
       for (auto& b : kv.second.slot)
         if (b.ptr) (void)hipFree(b.ptr);
       if (kv.second.stream) (void)hipStreamDestroy(kv.second.stream);
+      for (auto& st : kv.second.aux)
+        if (st) (void)hipStreamDestroy(st);
+      for (auto& e : kv.second.ev)
+        if (e) (void)hipEventDestroy(e);
       (void)hipSetDevice(cur);
     }
   }
@@ -73,6 +80,20 @@ hipStream_t host_api_stream() {
   DeviceState& ds = current();
   if (!ds.stream) ZH_CHECK(hipStreamCreateWithFlags(&ds.stream, hipStreamNonBlocking));
   return ds.stream;
+}
+
+hipStream_t aux_stream(int idx) {
+  DeviceState& ds = current();
+  hipStream_t& st = ds.aux[idx % kAux];
+  if (!st) ZH_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  return st;
+}
+
+hipEvent_t aux_event(int idx) {
+  DeviceState& ds = current();
+  hipEvent_t& e = ds.ev[idx % kEvents];
+  if (!e) ZH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return e;
 }
 
 int device_cu_count() {

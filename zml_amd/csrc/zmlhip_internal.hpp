@@ -82,6 +82,11 @@ void* workspace(size_t bytes, int slot = 0);
 // Stream used by the synchronous host-pointer entry points of this thread.
 hipStream_t host_api_stream();
 
+// Internal streams (default priority) and timing-free events of this thread
+// on the current device, used to overlap panel factorization with updates.
+hipStream_t aux_stream(int idx);
+hipEvent_t aux_event(int idx);
+
 // Number of compute units of the current device.
 int device_cu_count();
 
