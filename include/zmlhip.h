@@ -32,6 +32,18 @@ const char *zmlhip_last_error(void);
 /* Waits for all work queued on `stream`; returns 0 or ZMLHIP_EHIP. */
 int zmlhip_stream_synchronize(void *stream);
 
+/* Per-launch timing of the dominant kernels, for measurement tools. When
+ * enabled, HIP events are recorded around every launch of the kinds below on
+ * the stream it runs on; collect sums (and forgets) the records of one kind:
+ * total milliseconds, launches, algorithmic flops and bytes.
+ * No counterpart in the reference (zml has no GPU kernels to time). */
+#define ZMLHIP_PROF_GEMM_F64 1     /* FP64 trailing update of getrf */
+#define ZMLHIP_PROF_GEMM_LOWP 2    /* low-precision trailing update of the mixed LU */
+#define ZMLHIP_PROF_PANEL 3        /* panel factorization */
+int zmlhip_profile_enable(int on);
+int zmlhip_profile_collect(int kind, double *ms, long long *launches, double *flops,
+                           double *bytes);
+
 /* ---- BLAS (device pointers) ------------------------------------------- */
 /* cblas_dgemm (src/cblas.zig:526; blas/gemm.zig:14-1033) */
 int zmlhip_dgemm(int layout, int transa, int transb, int m, int n, int k, double alpha,
@@ -69,6 +81,35 @@ int zmlhip_dpotrf(int layout, char uplo, int n, double *a, int lda, int *info, v
 /* LAPACKE_dpotrs (src/linalg/lapack.zig:3466; lapack/potrs.zig:15-117) */
 int zmlhip_dpotrs(int layout, char uplo, int n, int nrhs, const double *a, int lda, double *b,
                   int ldb, void *stream);
+
+/* ---- mixed precision + iterative refinement ------------------------------ */
+/* No counterpart in the reference: zml has no mixed-precision factorization,
+ * refinement or GMRES. The result is the FP64 solution of the dgesv/dposv
+ * problem (src/linalg/lapack.zig:2051, :3896); A is not modified. */
+typedef struct zmlhip_ir_opts {
+  int precision;       /* trailing update: 0 = FP32, 1 = FP16 (default), 2 = BF16 */
+  int max_iter;        /* classic refinement steps before GMRES-IR (default 30) */
+  int gmres_restart;   /* FGMRES restart length (default 30) */
+  int gmres_max_iter;  /* total FGMRES iterations (default 300) */
+  int no_gmres;        /* 1: on stagnation go straight to the FP64 factorization */
+  double tol;          /* stop at ||b-Ax||inf <= tol ||A||inf ||x||inf n u, u = 2^-53 (default 1) */
+} zmlhip_ir_opts;
+
+typedef struct zmlhip_ir_result {
+  int info;              /* first zero pivot of the low-precision factorization, 0 if none */
+  int iterations;        /* classic refinement steps (largest over the right-hand sides) */
+  int gmres_iterations;  /* FGMRES iterations (largest over the right-hand sides) */
+  int fallback;          /* 0 none, 1 GMRES-IR was needed, 2 the FP64 factorization was used */
+  double backward_error; /* ||b-Ax||inf / (||A||inf ||x||inf n u), worst right-hand side */
+} zmlhip_ir_result;
+
+/* A x = b for general A (n x n, device, layout 101/102), nrhs right-hand sides
+ * b (ldb), solution into x (ldx, same layout). Synchronous with the host
+ * (returns when x is final); work is ordered on `stream`. opts may be NULL.
+ * Returns 0, -i for an invalid argument, or a ZMLHIP_E* code. */
+int zmlhip_dgesv_mixed(int layout, int n, int nrhs, const double *a, int lda, const double *b,
+                       int ldb, double *x, int ldx, const zmlhip_ir_opts *opts,
+                       zmlhip_ir_result *result, void *stream);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,8 @@ SIGNATURES = {
     "zmlhip_version": (_c_int, []),
     "zmlhip_last_error": (ctypes.c_char_p, []),
     "zmlhip_stream_synchronize": (_c_int, [_p]),
+    "zmlhip_profile_enable": (_c_int, [_c_int]),
+    "zmlhip_profile_collect": (_c_int, [_c_int, _p, _p, _p, _p]),
     "zmlhip_dgemm": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _p, _c_int,
                               _p, _c_int, _c_double, _p, _c_int, _p]),
     "zmlhip_dtrsm": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _p,
@@ -58,7 +60,23 @@ SIGNATURES = {
     "zmlhip_dgetrs": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _p, _c_int, _p]),
     "zmlhip_dpotrf": (_c_int, [_c_int, _c_char, _c_int, _p, _c_int, _p, _p]),
     "zmlhip_dpotrs": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _c_int, _p]),
+    "zmlhip_dgesv_mixed": (_c_int, [_c_int, _c_int, _c_int, _p, _c_int, _p, _c_int, _p, _c_int, _p, _p,
+                                    _p]),
 }
+
+
+class IROpts(ctypes.Structure):
+    """zmlhip_ir_opts (include/zmlhip.h)."""
+    _fields_ = [("precision", ctypes.c_int), ("max_iter", ctypes.c_int),
+                ("gmres_restart", ctypes.c_int), ("gmres_max_iter", ctypes.c_int),
+                ("no_gmres", ctypes.c_int), ("tol", ctypes.c_double)]
+
+
+class IRResult(ctypes.Structure):
+    """zmlhip_ir_result (include/zmlhip.h)."""
+    _fields_ = [("info", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("gmres_iterations", ctypes.c_int), ("fallback", ctypes.c_int),
+                ("backward_error", ctypes.c_double)]
 
 _lib = None
 
@@ -71,6 +89,13 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     if not os.path.exists(path):
         raise RuntimeError(
             f"{path} is missing: build it with `python -m zml_amd.build` (hipcc, gfx950)")
+    # One HIP runtime per process: when PyTorch is present it brings its own
+    # libamdhip64 (same soname); loading it first makes this library bind to
+    # that copy instead of starting a second runtime on the same device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(L, name)  # AttributeError when a symbol is not exported

@@ -12,6 +12,7 @@
 #include "../../include/cblas.h"
 #include "../../include/clapack.h"
 #include "../../include/zmlhip.h"
+#include "mixed.hpp"
 
 using namespace zmlhip;
 
@@ -285,6 +286,18 @@ int zmlhip_version(void) { return 100; }
 
 const char* zmlhip_last_error(void) { return last_error(); }
 
+int zmlhip_profile_enable(int on) {
+  profile_enable(on != 0);
+  return 0;
+}
+
+int zmlhip_profile_collect(int kind, double* ms, long long* launches, double* flops, double* bytes) {
+  return guarded([&] {
+    profile_collect(kind, ms, launches, flops, bytes);
+    return 0;
+  });
+}
+
 int zmlhip_stream_synchronize(void* stream) {
   return guarded([&] {
     ZH_CHECK(hipStreamSynchronize(as_stream(stream)));
@@ -373,6 +386,22 @@ int zmlhip_dpotrs(int layout, char uplo, int n, int nrhs, const double* a, int l
                   int ldb, void* stream) {
   return guarded([&] {
     return dpotrs_dev(layout, uplo, n, nrhs, a, lda, b, ldb, as_stream(stream));
+  });
+}
+
+int zmlhip_dgesv_mixed(int layout, int n, int nrhs, const double* a, int lda, const double* b, int ldb,
+                       double* x, int ldx, const zmlhip_ir_opts* opts, zmlhip_ir_result* result,
+                       void* stream) {
+  return guarded([&] {
+    if (!valid_layout(layout)) return -1;
+    if (n < 0) return -2;
+    if (nrhs < 0) return -3;
+    if (lda < std::max(1, n)) return -5;
+    if (ldb < std::max(1, layout == kColMajor ? n : nrhs)) return -7;
+    if (ldx < std::max(1, layout == kColMajor ? n : nrhs)) return -9;
+    if (opts && (opts->precision < 0 || opts->precision > 2)) return -10;
+    if (!result) return -11;
+    return dgesv_mixed(layout, n, nrhs, a, lda, b, ldb, x, ldx, opts, result, as_stream(stream));
   });
 }
 

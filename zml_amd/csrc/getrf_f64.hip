@@ -22,6 +22,7 @@
 // i.e. the very operation sequence of zml's unblocked closed form.
 #include "zmlhip_internal.hpp"
 #include "panel.hpp"
+#include "mixed.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -45,19 +46,8 @@ static void trsm_rm_llnu(int64_t m, int64_t n, const double* a, int64_t lda, dou
 // Trailing-update kernels of a row-major LU: C -= L U and B <- L11^-1 B.
 // The wide U12 solve goes through the inverse of the unit lower L11: the
 // panel stream forms L11^-1 right after factoring the block column, and the
-// main stream then computes U12 = L11^-1 A12 as one GEMM on a copy of A12.
-template <typename T>
-struct LuOps {
-  void (*gemm)(int64_t rows, int64_t cols, int64_t kb, const T* L, int64_t ldl, const T* U,
-               int64_t ldu, T* C, int64_t ldc, hipStream_t s);
-  void (*trsm)(int64_t kb, int64_t cols, const T* L11, int64_t ld, T* B, int64_t ldb,
-               hipStream_t s);
-  // inv (kb x kb, ld kb) <- L11^-1
-  void (*invert)(int64_t kb, const T* L11, int64_t ld, T* inv, hipStream_t s);
-  // B (kb x cols) <- inv * B, through tmp (kb x cols, ld cols)
-  void (*apply_inv)(int64_t kb, int64_t cols, const T* inv, T* B, int64_t ldb, T* tmp,
-                    hipStream_t s);
-};
+// main stream then computes U12 = L11^-1 A12 as one GEMM on a copy of A12
+// (LuOps is declared in mixed.hpp, shared with the mixed-precision driver).
 
 struct LeafState {
   void* ws = nullptr;
@@ -150,8 +140,9 @@ void factor_panel(int64_t m, T* A, int64_t lda, int64_t j0, int64_t jb, int32_t*
     if (rc > 0) {
       ops.trsm(kb, rc, A + k0 * lda + k0, lda, A + k0 * lda + k0 + kb, lda, s);
       if (m - k0 - kb > 0)
-        ops.gemm(m - k0 - kb, rc, kb, A + (k0 + kb) * lda + k0, lda, A + k0 * lda + k0 + kb, lda,
-                 A + (k0 + kb) * lda + k0 + kb, lda, s);
+        (ops.panel_gemm ? ops.panel_gemm : ops.gemm)(m - k0 - kb, rc, kb, A + (k0 + kb) * lda + k0,
+                                                     lda, A + k0 * lda + k0 + kb, lda,
+                                                     A + (k0 + kb) * lda + k0 + kb, lda, s);
     }
   }
 }
@@ -209,7 +200,10 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
 
 static void gemm_f64_update(int64_t rows, int64_t cols, int64_t kb, const double* L, int64_t ldl,
                             const double* U, int64_t ldu, double* C, int64_t ldc, hipStream_t s) {
+  void* tok = profile_begin(kProfGemmF64, s);
   gemm_rm_nn(rows, cols, kb, -1.0, L, ldl, U, ldu, 1.0, C, ldc, s);
+  // algorithmic traffic: L and U read once, C read and written once
+  profile_end(tok, s, 2.0 * rows * cols * kb, 8.0 * (rows * kb + kb * cols + 2.0 * rows * cols));
 }
 static void trsm_f64_update(int64_t kb, int64_t cols, const double* L11, int64_t ld, double* B,
                             int64_t ldb, hipStream_t s) {
@@ -234,9 +228,12 @@ static void apply_inv_f64(int64_t kb, int64_t cols, const double* inv, double* B
 
 void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, int* info,
                   hipStream_t s) {
-  const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64};
+  const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64, nullptr};
   getrf_lookahead<double>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }
+
+template void getrf_lookahead<float>(int64_t, int64_t, float*, int64_t, int32_t*, int*, int64_t,
+                                     const LuOps<float>&, hipStream_t);
 
 int leaf_error_flag(hipStream_t s) {
   void* ws = workspace(panel::ws_bytes(), 3);

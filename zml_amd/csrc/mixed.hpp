@@ -1,0 +1,50 @@
+// Declarations shared by the mixed-precision sources (mp_gemm.hip, mixed.hip).
+#pragma once
+
+#include "zmlhip_internal.hpp"
+#include "../../include/zmlhip.h"
+
+namespace zmlhip {
+
+// C = beta C + alpha rs_i cs_j sum_k A_ik B_kj, FP32 accumulate (mp_gemm.hip).
+void mp_gemm_f16(int64_t m, int64_t n, int64_t K, float alpha, const _Float16* A, int64_t lda,
+                 const _Float16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
+                 float* C, int64_t ldc, hipStream_t s);
+void mp_gemm_bf16(int64_t m, int64_t n, int64_t K, float alpha, const __bf16* A, int64_t lda,
+                  const __bf16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
+                  float* C, int64_t ldc, hipStream_t s);
+void mp_gemm_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, int64_t lda,
+                 const float* B, int64_t ldb, float beta, const float* rs, const float* cs, float* C,
+                 int64_t ldc, hipStream_t s);
+
+// Precision of the trailing update of the mixed factorizations.
+enum LowPrec : int { kPrecF32 = 0, kPrecF16 = 1, kPrecBF16 = 2 };
+
+// Row-major LU machinery shared with the FP64 driver (getrf_f64.hip).
+template <typename T>
+struct LuOps {
+  // trailing update C -= L U (rows x cols, inner kb)
+  void (*gemm)(int64_t rows, int64_t cols, int64_t kb, const T* L, int64_t ldl, const T* U,
+               int64_t ldu, T* C, int64_t ldc, hipStream_t s);
+  // B (kb x cols) <- L11^-1 B, L11 unit lower (kb <= 64), inside a block column
+  void (*trsm)(int64_t kb, int64_t cols, const T* L11, int64_t ld, T* B, int64_t ldb,
+               hipStream_t s);
+  // inv (kb x kb, ld kb) <- L11^-1 for the block column's unit lower L11
+  void (*invert)(int64_t kb, const T* L11, int64_t ld, T* inv, hipStream_t s);
+  // B (kb x cols) <- inv * B, through tmp (kb x cols, ld cols)
+  void (*apply_inv)(int64_t kb, int64_t cols, const T* inv, T* B, int64_t ldb, T* tmp,
+                    hipStream_t s);
+  // update inside a block column (C -= L U); the trailing update when null
+  void (*panel_gemm)(int64_t rows, int64_t cols, int64_t kb, const T* L, int64_t ldl, const T* U,
+                     int64_t ldu, T* C, int64_t ldc, hipStream_t s);
+};
+
+int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,
+                int64_t ldb, double* X, int64_t ldx, const zmlhip_ir_opts* o, zmlhip_ir_result* res,
+                hipStream_t s);
+
+template <typename T>
+void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info, int64_t nb,
+                     const LuOps<T>& ops, hipStream_t s);
+
+}  // namespace zmlhip

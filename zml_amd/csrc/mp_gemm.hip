@@ -1,0 +1,285 @@
+// Low-precision GEMMs of the mixed-precision factorizations, FP32 accumulate.
+//
+//   C = beta * C + alpha * rs_i * cs_j * sum_k A_ik B_kj        (row-major)
+//
+// C is FP32. rs / cs are optional per-row / per-column multipliers (the
+// inverse power-of-two scales applied when the operands were rounded to
+// FP16/BF16, so that A_ik * rs_i and B_kj * cs_j stayed in range).
+//   * FP16 / BF16: v_mfma_f32_32x32x16_{f16,bf16}. Both operands k-contiguous:
+//     A is m x K row-major, B is given transposed (BT: n x K row-major).
+//   * FP32: v_mfma_f32_32x32x2_f32. A is m x K row-major, B is K x n
+//     row-major (its natural layout: one element per lane per MFMA).
+// A 256-thread workgroup computes 128 x 128 of C (2 x 2 waves of 64 x 64,
+// each 2 x 2 MFMA blocks of 32 x 32); K advances through double-buffered
+// LDS. Lane l of a 32 x 32 block holds column l % 32 and rows
+// 8 (r / 4) + 4 ((l / 32) % 2) + r % 4 of register r, so every C access is
+// 32 consecutive floats.
+#include "zmlhip_internal.hpp"
+#include "mixed.hpp"
+
+namespace zmlhip {
+namespace {
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+
+constexpr int MT = 128, NTL = 128, NTH = 256;
+
+__device__ __forceinline__ void tile_of(int64_t bid, int64_t ntm, int64_t ntn, int64_t& tm, int64_t& tn) {
+  const int64_t nt = ntm * ntn;
+  const int64_t x = bid & 7, q = nt >> 3, r8 = nt & 7;
+  const int64_t pid = x * q + (x < r8 ? x : r8) + (bid >> 3);
+  constexpr int64_t G = 8;
+  const int64_t per = G * ntn, grp = pid / per, fm = grp * G;
+  const int64_t gm = (ntm - fm) < G ? (ntm - fm) : G;
+  const int64_t r = pid - grp * per;
+  tm = fm + r % gm;
+  tn = r / gm;
+}
+
+__device__ __forceinline__ void epilogue(const f16v (&acc)[2][2], int64_t m, int64_t n, float alpha,
+                                         float beta, const float* __restrict__ rs,
+                                         const float* __restrict__ cs, float* __restrict__ C,
+                                         int64_t ldc, int64_t r0, int64_t c0, int lane) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int64_t j = c0 + 32 * bj + (lane & 31);
+      if (j >= n) continue;
+      const float cj = cs ? cs[j] : 1.0f;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = r0 + 32 * bi + 8 * (r >> 2) + 4 * ((lane >> 5) & 1) + (r & 3);
+        if (i >= m) continue;
+        const float s = alpha * cj * (rs ? rs[i] : 1.0f);
+        float* p = C + i * ldc + j;
+        *p = beta == 0.0f ? s * acc[bi][bj][r] : beta * *p + s * acc[bi][bj][r];
+      }
+    }
+}
+
+// ---- FP16 / BF16 ---------------------------------------------------------------
+template <typename H>
+struct MfmaH;
+template <> struct MfmaH<_Float16> {
+  typedef h8 vec;
+  static __device__ __forceinline__ f16v mma(vec a, vec b, f16v c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct MfmaH<__bf16> {
+  typedef b8 vec;
+  static __device__ __forceinline__ f16v mma(vec a, vec b, f16v c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int BKH = 32, PADH = 8, LDH = BKH + PADH;   // halfs per LDS row
+
+template <typename H>
+__global__ void __launch_bounds__(NTH, 2)
+mp_gemm_h_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restrict__ A, int64_t lda,
+                 const H* __restrict__ BT, int64_t ldb, float beta, const float* __restrict__ rs,
+                 const float* __restrict__ cs, float* __restrict__ C, int64_t ldc) {
+  typedef typename MfmaH<H>::vec V;
+  __shared__ __attribute__((aligned(16))) H As[2][MT * LDH];
+  __shared__ __attribute__((aligned(16))) H Bs[2][NTL * LDH];
+  const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
+  int64_t tm, tn;
+  tile_of(blockIdx.x, ntm, ntn, tm, tn);
+  const int64_t row0 = tm * MT, col0 = tn * NTL;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // each thread stages 2 x 16 bytes of A and of B per k-slab: row t / 4 (+64), k chunk t % 4
+  uint4 ra[2], rb[2];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int r = (t >> 2) + 64 * p, kc = (t & 3) * 8;
+      const int64_t kk = k0 + kc;
+      const int64_t ia = row0 + r, jb = col0 + r;
+      uint4 va = {0, 0, 0, 0}, vb = {0, 0, 0, 0};
+      if (ia < m) {
+        if (kk + 8 <= K) va = *reinterpret_cast<const uint4*>(A + ia * lda + kk);
+        else {
+          H tmp[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) tmp[e] = (kk + e < K) ? A[ia * lda + kk + e] : H(0.0f);
+          va = *reinterpret_cast<uint4*>(tmp);
+        }
+      }
+      if (jb < n) {
+        if (kk + 8 <= K) vb = *reinterpret_cast<const uint4*>(BT + jb * ldb + kk);
+        else {
+          H tmp[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) tmp[e] = (kk + e < K) ? BT[jb * ldb + kk + e] : H(0.0f);
+          vb = *reinterpret_cast<uint4*>(tmp);
+        }
+      }
+      ra[p] = va;
+      rb[p] = vb;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int r = (t >> 2) + 64 * p, kc = (t & 3) * 8;
+      *reinterpret_cast<uint4*>(&As[buf][r * LDH + kc]) = ra[p];
+      *reinterpret_cast<uint4*>(&Bs[buf][r * LDH + kc]) = rb[p];
+    }
+  };
+
+  f16v acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+
+  const int nk = (int)((K + BKH - 1) / BKH);
+  load(0);
+  store(0);
+  __syncthreads();
+  const int li = lane & 31, lk = (lane >> 5) * 8;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((int64_t)(kt + 1) * BKH);
+#pragma unroll
+    for (int ks = 0; ks < BKH / 16; ++ks) {
+      V av[2], bv[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+        av[x] = *reinterpret_cast<const V*>(&As[cur][(wm * 64 + 32 * x + li) * LDH + 16 * ks + lk]);
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+        bv[y] = *reinterpret_cast<const V*>(&Bs[cur][(wn * 64 + 32 * y + li) * LDH + 16 * ks + lk]);
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = MfmaH<H>::mma(av[x], bv[y], acc[x][y]);
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+  epilogue(acc, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
+}
+
+// ---- FP32 ------------------------------------------------------------------------
+constexpr int BKF = 16;
+constexpr int LDAF = BKF + 1;        // A rows in LDS: [i][k], padded
+constexpr int LDBF = NTL;            // B rows in LDS: [k][j]
+
+__global__ void __launch_bounds__(NTH, 2)
+mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __restrict__ A,
+                 int64_t lda, const float* __restrict__ B, int64_t ldb, float beta,
+                 const float* __restrict__ rs, const float* __restrict__ cs, float* __restrict__ C,
+                 int64_t ldc) {
+  __shared__ float As[2][MT * LDAF];
+  __shared__ float Bs[2][BKF * LDBF];
+  const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
+  int64_t tm, tn;
+  tile_of(blockIdx.x, ntm, ntn, tm, tn);
+  const int64_t row0 = tm * MT, col0 = tn * NTL;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // A tile 128 x 16: thread -> row t / 2 (8 floats); B tile 16 x 128: row t / 16 (8 floats)
+  float ra[8], rb[8];
+  auto load = [&](int64_t k0) {
+    const int r = t >> 1, kc = (t & 1) * 8;
+    const int64_t ia = row0 + r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t kk = k0 + kc + e;
+      ra[e] = (ia < m && kk < K) ? A[ia * lda + kk] : 0.0f;
+    }
+    const int kr = t >> 4, jc = (t & 15) * 8;
+    const int64_t kk = k0 + kr;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t j = col0 + jc + e;
+      rb[e] = (kk < K && j < n) ? B[kk * ldb + j] : 0.0f;
+    }
+  };
+  auto store = [&](int buf) {
+    const int r = t >> 1, kc = (t & 1) * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) As[buf][r * LDAF + kc + e] = ra[e];
+    const int kr = t >> 4, jc = (t & 15) * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) Bs[buf][kr * LDBF + jc + e] = rb[e];
+  };
+
+  f16v acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+
+  const int nk = (int)((K + BKF - 1) / BKF);
+  load(0);
+  store(0);
+  __syncthreads();
+  const int li = lane & 31, lk = lane >> 5;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((int64_t)(kt + 1) * BKF);
+#pragma unroll
+    for (int ks = 0; ks < BKF / 2; ++ks) {
+      float av[2], bv[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) av[x] = As[cur][(wm * 64 + 32 * x + li) * LDAF + 2 * ks + lk];
+#pragma unroll
+      for (int y = 0; y < 2; ++y) bv[y] = Bs[cur][(2 * ks + lk) * LDBF + wn * 64 + 32 * y + li];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+  epilogue(acc, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
+}
+
+}  // namespace
+
+void mp_gemm_f16(int64_t m, int64_t n, int64_t K, float alpha, const _Float16* A, int64_t lda,
+                 const _Float16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
+                 float* C, int64_t ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return;
+  const int64_t tiles = ((m + MT - 1) / MT) * ((n + NTL - 1) / NTL);
+  hipLaunchKernelGGL((mp_gemm_h_kernel<_Float16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
+                     A, lda, BT, ldb, beta, rs, cs, C, ldc);
+  ZH_LAUNCH_CHECK();
+}
+
+void mp_gemm_bf16(int64_t m, int64_t n, int64_t K, float alpha, const __bf16* A, int64_t lda,
+                  const __bf16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
+                  float* C, int64_t ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return;
+  const int64_t tiles = ((m + MT - 1) / MT) * ((n + NTL - 1) / NTL);
+  hipLaunchKernelGGL((mp_gemm_h_kernel<__bf16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
+                     A, lda, BT, ldb, beta, rs, cs, C, ldc);
+  ZH_LAUNCH_CHECK();
+}
+
+void mp_gemm_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, int64_t lda,
+                 const float* B, int64_t ldb, float beta, const float* rs, const float* cs, float* C,
+                 int64_t ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return;
+  const int64_t tiles = ((m + MT - 1) / MT) * ((n + NTL - 1) / NTL);
+  hipLaunchKernelGGL(mp_gemm_f_kernel, dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha, A, lda,
+                     B, ldb, beta, rs, cs, C, ldc);
+  ZH_LAUNCH_CHECK();
+}
+
+}  // namespace zmlhip

@@ -5,6 +5,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 namespace zmlhip {
 
@@ -15,7 +16,7 @@ struct Buf {
   void* ptr = nullptr;
   size_t bytes = 0;
 };
-constexpr int kSlots = 8;
+constexpr int kSlots = 16;
 constexpr int kAux = 4, kEvents = 8;
 struct DeviceState {
   Buf slot[kSlots];
@@ -51,6 +52,52 @@ DeviceState& current() {
 }  // namespace
 
 void set_last_error(const std::string& msg) { tl_last_error = msg; }
+
+// ---- optional timing of the dominant kernels (HIP events on their streams) ----
+namespace {
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+  double flops, bytes;
+};
+thread_local bool tl_prof_on = false;
+thread_local std::vector<ProfRec> tl_prof;
+}  // namespace
+
+bool profiling() { return tl_prof_on; }
+void profile_enable(bool on) { tl_prof_on = on; }
+void* profile_begin(int kind, hipStream_t s) {
+  if (!tl_prof_on) return nullptr;
+  ProfRec r{kind, nullptr, nullptr, 0.0, 0.0};
+  ZH_CHECK(hipEventCreate(&r.a));
+  ZH_CHECK(hipEventCreate(&r.b));
+  ZH_CHECK(hipEventRecord(r.a, s));
+  tl_prof.push_back(r);
+  return reinterpret_cast<void*>(tl_prof.size());
+}
+void profile_end(void* token, hipStream_t s, double flops, double bytes) {
+  if (!token) return;
+  ProfRec& r = tl_prof[reinterpret_cast<size_t>(token) - 1];
+  ZH_CHECK(hipEventRecord(r.b, s));
+  r.flops = flops;
+  r.bytes = bytes;
+}
+void profile_collect(int kind, double* ms, long long* launches, double* flops, double* bytes) {
+  double t = 0, f = 0, b = 0;
+  long long c = 0;
+  std::vector<ProfRec> keep;
+  for (auto& r : tl_prof) {
+    if (r.kind != kind) { keep.push_back(r); continue; }
+    ZH_CHECK(hipEventSynchronize(r.b));
+    float e = 0.f;
+    ZH_CHECK(hipEventElapsedTime(&e, r.a, r.b));
+    t += e; f += r.flops; b += r.bytes; ++c;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  tl_prof.swap(keep);
+  *ms = t; *launches = c; *flops = f; *bytes = b;
+}
 const char* last_error() { return tl_last_error.c_str(); }
 
 void* workspace(size_t bytes, int slot) {

@@ -87,6 +87,15 @@ hipStream_t host_api_stream();
 hipStream_t aux_stream(int idx);
 hipEvent_t aux_event(int idx);
 
+// Optional per-launch timing of the dominant kernels (kinds below), with HIP
+// events recorded on the stream each kernel is launched on.
+enum ProfKind : int { kProfGemmF64 = 1, kProfGemmLowp = 2, kProfPanel = 3 };
+bool profiling();
+void profile_enable(bool on);
+void* profile_begin(int kind, hipStream_t s);
+void profile_end(void* token, hipStream_t s, double flops, double bytes);
+void profile_collect(int kind, double* ms, long long* launches, double* flops, double* bytes);
+
 // Number of compute units of the current device.
 int device_cu_count();
 
