@@ -1,0 +1,219 @@
+"""Headline benchmark: LU solve TFLOP/s at FP64 backward error, mixed
+precision + iterative refinement versus plain FP64, n = 32768, on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--delta D] [--precision fp16|bf16|fp32]
+
+Workload (the mixed-precision LU configuration): A = diag(r) (R + delta n I)
+diag(c), R ~ U[-0.5, 0.5], r, c = 10^U(-3, 3), one right-hand side b = A x_true,
+synthetic and generated on the device; A and b are resident in HBM before
+timing starts. One step is one complete solve through the C ABI entry point
+zmlhip_dgesv_mixed: equilibration, FP32/low-precision LU, triangular solves
+and FP64 refinement until ||b - Ax||_inf <= ||A||_inf ||x||_inf n u. A step
+only counts if the result meets the FP64 backward-error bar (<= 16); the
+throughput is 2/3 n^3 flops per step.
+
+The plain FP64 path (zmlhip_dgetrf + zmlhip_dgetrs on a copy of A) is timed
+on the same system for the speed-up. The roofline object describes the
+dominant kernel, the low-precision trailing-update GEMM, from HIP events the
+library records around each of its launches on the stream it runs on. The
+CPU baseline is zml's own algorithm (the C restatement of getrf/getrs in
+oracle/) on a bounded sample, one core.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LU solve TFLOP/s at FP64 backward error, mixed+IR vs plain FP64, n=32768"
+PEAK = {"fp16": 2500.0, "bf16": 2500.0, "fp32": 157.3, "fp64": 78.6}   # dense TFLOP/s, MI355X
+PREC = {"fp32": 0, "fp16": 1, "bf16": 2}
+PROF_GEMM_F64, PROF_GEMM_LOWP = 1, 2
+
+
+def cpu_baseline(n=2048, reps=2):
+    """zml's getrf + getrs (oracle/zml_oracle.c restatement), one core."""
+    import numpy as np
+    import oracle as O
+    O.lib()
+    best = None
+    for _ in range(reps):
+        a = O.fill_uniform(O.ROW, n, n, 42)
+        b = O.fill_uniform(O.ROW, n, 1, 43)
+        ip = np.zeros(n, np.int32)
+        t0 = time.perf_counter()
+        assert O.getrf(O.ROW, n, n, a, n, ip) == 0
+        O.getrs(O.ROW, 111, n, 1, a, n, ip, b, 1)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return {"value": round((2.0 / 3.0) * n ** 3 / best / 1e12, 6), "unit": "TFLOP/s", "cores": 1,
+            "kind": "port",
+            "sample": f"zml getrf+getrs restated in C (oracle/zml_oracle.c, NB=64, no FMA), "
+                      f"n={n} row-major uniform[-1,1], 1 RHS, best of {reps}, {best:.2f} s"}
+
+
+def traffic_from_profiles(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--delta", type=float, default=0.1)
+    ap.add_argument("--precision", default="fp16", choices=sorted(PREC))
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the dense factor-and-solve path is built and timed on a "
+              "single MI355X (the multi-GPU layout is described in DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from zml_amd import _lib
+    from zml_amd._lib import IROpts, IRResult
+    zh = _lib.load()
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    n = args.n
+
+    # ---- synthetic system (device-generated, FP64)
+    g = torch.Generator(device=dev)
+    g.manual_seed(45)
+    A = torch.rand(n, n, dtype=torch.float64, device=dev, generator=g).sub_(0.5)
+    A.diagonal().add_(args.delta * n)
+    r = torch.pow(10.0, torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 6 - 3)
+    c = torch.pow(10.0, torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 6 - 3)
+    A.mul_(r[:, None]).mul_(c[None, :])
+    del r, c
+    x_true = torch.rand(n, 1, dtype=torch.float64, device=dev, generator=g).mul_(2).sub_(1)
+    b = A @ x_true
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    anorm = A.abs().sum(1).max().item()
+
+    def backward_error(xv):
+        res = b - A @ xv
+        return (res.abs().max() / (anorm * xv.abs().max() * n * 2.0 ** -53)).item()
+
+    # ---- mixed precision + IR (row-major view of A: layout 101)
+    opts = IROpts(precision=PREC[args.precision])
+    rres = IRResult()
+
+    def mixed_step():
+        rc = zh.zmlhip_dgesv_mixed(101, n, 1, P(A), n, P(b), 1, P(x), 1, ctypes.byref(opts),
+                                   ctypes.byref(rres), sh)
+        if rc != 0:
+            raise RuntimeError(f"zmlhip_dgesv_mixed: {rc} {_lib.last_error()}")
+
+    for _ in range(args.warmup):
+        mixed_step()
+    zh.zmlhip_profile_enable(1)
+    zh.zmlhip_profile_collect(PROF_GEMM_LOWP, *(ctypes.byref(v) for v in
+                              (ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double())))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters, gm, fb = [], [], []
+    for _ in range(args.steps):
+        mixed_step()
+        iters.append(rres.iterations)
+        gm.append(rres.gmres_iterations)
+        fb.append(rres.fallback)
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    zh.zmlhip_profile_enable(0)
+    ms, nl, fl, by = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+    zh.zmlhip_profile_collect(PROF_GEMM_LOWP, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(fl),
+                              ctypes.byref(by))
+    be_mixed = backward_error(x)
+    err_true = ((x - x_true).abs().max() / x_true.abs().max()).item()
+    flops = (2.0 / 3.0) * n ** 3
+    valid = be_mixed <= 16.0
+
+    # ---- plain FP64 on the same system
+    Aw = torch.empty_like(A)
+    ipiv = torch.zeros(n, dtype=torch.int32, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    x64 = torch.empty_like(b)
+
+    def fp64_step():
+        Aw.copy_(A)
+        x64.copy_(b)
+        rc = zh.zmlhip_dgetrf(101, n, n, P(Aw), n, P(ipiv), P(info), sh)
+        rc |= zh.zmlhip_dgetrs(101, b"N", n, 1, P(Aw), n, P(ipiv), P(x64), 1, sh)
+        if rc != 0:
+            raise RuntimeError(f"FP64 path: {_lib.last_error()}")
+
+    fp64_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        fp64_step()
+    torch.cuda.synchronize()
+    t_fp64 = (time.perf_counter() - t0) / args.fp64_steps
+    be_fp64 = backward_error(x64)
+    del Aw
+
+    prec = args.precision
+    achieved = (fl.value / (ms.value / 1e3) / 1e12) if ms.value > 0 else None
+    roofline = {
+        "bound": "mfma",
+        "kernel": f"mp_gemm ({prec} inputs, FP32 accumulate) trailing update",
+        "achieved": round(achieved, 2) if achieved else None,
+        "peak": PEAK[prec],
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK[prec], 4) if achieved else None,
+        "traffic": traffic_from_profiles(f"traffic_mp_gemm_{prec}_n{n}.json"),
+        "launches_per_step": nl.value // max(1, args.steps),
+        "avg_launch_ms": round(ms.value / max(1, nl.value), 4),
+        "algorithmic_bytes_per_launch": round(by.value / max(1, nl.value)),
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(flops / t_mixed / 1e12, 3) if valid else None,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_mixed * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{prec} update / fp32 panels / fp64 refinement",
+        "data": "synthetic (device-generated A = diag(r)(R + delta n I)diag(c), b = A x_true)",
+        "config": {"workload": f"dgesv mixed+IR, n={n}, delta={args.delta}, 1 RHS, row-major",
+                   "n": n, "delta": args.delta, "precision": prec, "nrhs": 1},
+        "backward_error": round(be_mixed, 4),
+        "error_vs_x_true": err_true,
+        "ir_iterations": iters, "gmres_iterations": gm, "fallback": fb,
+        "plain_fp64": {"ms": round(t_fp64 * 1e3, 2), "tflops": round(flops / t_fp64 / 1e12, 3),
+                       "backward_error": round(be_fp64, 4),
+                       "frac_fp64_peak": round(flops / t_fp64 / 1e12 / PEAK["fp64"], 4)},
+        "speedup_vs_plain_fp64": round(t_fp64 / t_mixed, 3),
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
