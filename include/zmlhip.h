@@ -93,6 +93,9 @@ typedef struct zmlhip_ir_opts {
   int gmres_max_iter;  /* total FGMRES iterations (default 300) */
   int no_gmres;        /* 1: on stagnation go straight to the FP64 factorization */
   double tol;          /* stop at ||b-Ax||inf <= tol ||A||inf ||x||inf n u, u = 2^-53 (default 1) */
+  int no_optimistic;   /* 1: every block column takes the full pivot search (no diagonal-block try) */
+  double pivot_threshold; /* diagonal-block try accepted when every |l_ij| below the block is <= this
+                             (default 16; 1 keeps exactly the partial-pivoting choice) */
 } zmlhip_ir_opts;
 
 typedef struct zmlhip_ir_result {
@@ -101,6 +104,8 @@ typedef struct zmlhip_ir_result {
   int gmres_iterations;  /* FGMRES iterations (largest over the right-hand sides) */
   int fallback;          /* 0 none, 1 GMRES-IR was needed, 2 the FP64 factorization was used */
   double backward_error; /* ||b-Ax||inf / (||A||inf ||x||inf n u), worst right-hand side */
+  int panels;            /* block columns tried with pivots inside the diagonal block */
+  int panels_optimistic; /* ... of which needed no pivot from below the block */
 } zmlhip_ir_result;
 
 /* A x = b for general A (n x n, device, layout 101/102), nrhs right-hand sides

@@ -81,10 +81,13 @@ def test_mixed_lu_config3_backward_error(zh, torch_cuda, precision, order, delta
     fe = np.abs(X - Xo).max() / np.abs(Xo).max()
     print(f"prec={precision} order={order} delta={delta}: backward error {be:.3g} (n u units), "
           f"lib {res.backward_error:.3g}, IR {res.iterations}, GMRES {res.gmres_iterations}, "
-          f"fallback {res.fallback}, error vs oracle {fe:.3g}")
+          f"fallback {res.fallback}, error vs oracle {fe:.3g}, "
+          f"diagonal-block panels {res.panels_optimistic}/{res.panels}")
     assert res.info == 0
     assert be <= 16 and res.backward_error <= 16
     assert res.fallback < 2
+    # these matrices are diagonally heavy after equilibration: no pivot comes from below a block
+    assert res.panels == 3 and res.panels_optimistic == 3
 
 
 @pytest.mark.parametrize("precision", [0, 1, 2])
@@ -110,6 +113,34 @@ def test_mixed_lu_random_uniform_needs_pivoting(zh, torch_cuda):
         assert backward_error(A, X, B) <= 16, res.backward_error
         Xo = oracle_solve(A, B)
         assert np.abs(X - Xo).max() / np.abs(Xo).max() < 1e-9
+        assert res.panels == 4 and res.panels_optimistic < 4
+    # at threshold 1 (only partial pivoting's own choices) every diagonal-block try
+    # with rows below it is rejected and redone; the last block (no rows below) is
+    # exact either way
+    X, res = run_mixed(zh, torch_cuda, A, B, COL, 1, pivot_threshold=1.0)
+    assert res.panels == 4 and res.panels_optimistic == 1
+    assert backward_error(A, X, B) <= 16
+
+
+@pytest.mark.parametrize("mode", ["threshold16", "exact", "full_search"])
+def test_mixed_lu_panel_paths_agree(zh, torch_cuda, mode):
+    """The same system through diagonal-block panels (threshold 16: all accepted;
+    threshold 1, i.e. only where partial pivoting would agree: some rejected and
+    redone) and through full pivot searches only."""
+    n = 1280
+    A = config3_matrix(n, 0.1, 21)
+    X0 = O.fill_uniform(ROW, n, 1, 44).reshape(n, 1)
+    B = A @ X0
+    kw = {"threshold16": {}, "exact": {"pivot_threshold": 1.0}, "full_search": {"no_optimistic": 1}}[mode]
+    X, res = run_mixed(zh, torch_cuda, A, B, ROW, 1, **kw)
+    print(f"{mode}: diagonal-block panels {res.panels_optimistic}/{res.panels}, IR {res.iterations}, "
+          f"GMRES {res.gmres_iterations}")
+    assert res.panels == (0 if mode == "full_search" else 5)
+    if mode == "threshold16":
+        assert res.panels_optimistic == 5
+    if mode == "exact":
+        assert 1 <= res.panels_optimistic < 5   # the last block has no rows below it
+    assert backward_error(A, X, B) <= 16
 
 
 @pytest.mark.parametrize("order", [ROW, COL])

@@ -69,14 +69,16 @@ class IROpts(ctypes.Structure):
     """zmlhip_ir_opts (include/zmlhip.h)."""
     _fields_ = [("precision", ctypes.c_int), ("max_iter", ctypes.c_int),
                 ("gmres_restart", ctypes.c_int), ("gmres_max_iter", ctypes.c_int),
-                ("no_gmres", ctypes.c_int), ("tol", ctypes.c_double)]
+                ("no_gmres", ctypes.c_int), ("tol", ctypes.c_double), ("no_optimistic", ctypes.c_int),
+                ("pivot_threshold", ctypes.c_double)]
 
 
 class IRResult(ctypes.Structure):
     """zmlhip_ir_result (include/zmlhip.h)."""
     _fields_ = [("info", ctypes.c_int), ("iterations", ctypes.c_int),
                 ("gmres_iterations", ctypes.c_int), ("fallback", ctypes.c_int),
-                ("backward_error", ctypes.c_double)]
+                ("backward_error", ctypes.c_double), ("panels", ctypes.c_int),
+                ("panels_optimistic", ctypes.c_int)]
 
 _lib = None
 

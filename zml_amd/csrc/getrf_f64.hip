@@ -163,7 +163,8 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
   T* inv = static_cast<T*>(workspace(sizeof(T) * 2 * nb * nb, 0));
   T* tmp = static_cast<T*>(workspace(sizeof(T) * nb * (size_t)std::max<int64_t>(n, 1), 1));
   auto panel_and_inverse = [&](int64_t j0, int64_t jb, int64_t blk) {
-    factor_panel<T>(m, A, lda, j0, jb, ipiv, info, ops, sp, 2);
+    if (!(ops.try_panel && ops.try_panel(m, A, lda, j0, jb, ipiv, sp)))
+      factor_panel<T>(m, A, lda, j0, jb, ipiv, info, ops, sp, 2);
     if (j0 + jb < n) ops.invert(jb, A + j0 * lda + j0, lda, inv + (blk & 1) * nb * nb, sp);
     ZH_CHECK(hipEventRecord(ev_panel, sp));
   };
@@ -184,16 +185,19 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
       ops.apply_inv(jb, jb2, invb, A + j0 * lda + c1, lda, tmp, s);
       if (rows > 0) ops.gemm(rows, jb2, jb, A + c1 * lda + j0, lda, A + j0 * lda + c1, lda, A + c1 * lda + c1, lda, s);
       ZH_CHECK(hipEventRecord(ev_next, s));
-      ZH_CHECK(hipStreamWaitEvent(sp, ev_next, 0));
-      panel_and_inverse(c1, jb2, blk + 1);
     }
-    // the rest: interchanges left of the block and beyond the next one, then update
+    // the rest: interchanges left of the block and beyond the next one, then
+    // update (queued before the next panel: an optimistic panel waits on the host)
     apply_perm<T>(j0, A, lda, pr, s);
     const int64_t c2 = c1 + jb2, nr = n - c2;
     if (nr > 0) {
       apply_perm<T>(nr, A + c2, lda, pr, s);
       ops.apply_inv(jb, nr, invb, A + j0 * lda + c2, lda, tmp, s);
       if (rows > 0) ops.gemm(rows, nr, jb, A + c1 * lda + j0, lda, A + j0 * lda + c2, lda, A + c1 * lda + c2, lda, s);
+    }
+    if (jb2 > 0) {
+      ZH_CHECK(hipStreamWaitEvent(sp, ev_next, 0));
+      panel_and_inverse(c1, jb2, blk + 1);
     }
   }
 }
@@ -228,7 +232,7 @@ static void apply_inv_f64(int64_t kb, int64_t cols, const double* inv, double* B
 
 void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, int* info,
                   hipStream_t s) {
-  const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64, nullptr};
+  const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64, nullptr, nullptr};
   getrf_lookahead<double>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }
 

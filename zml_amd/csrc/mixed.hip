@@ -52,36 +52,32 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
 }
 
-// rmax[i] = max_j |a_ij| (rs == null) or max over columns of |rs_i a_ij| into cmax[j].
-__global__ void absmax_kernel(int layout, int64_t n, const double* __restrict__ a, int64_t lda,
-                              const double* __restrict__ rscale, double* __restrict__ rmax,
-                              double* __restrict__ cmax, int64_t chunk) {
-  // element (i, j); threads walk the contiguous index, blocks.y split the other
-  const int64_t fast = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (fast >= n) return;
-  const int64_t s0 = blockIdx.y * chunk, s1 = std::min<int64_t>(n, s0 + chunk);
+// A "line" is a row of a row-major matrix or a column of a column-major one
+// (its elements are contiguous); a "position" indexes along a line.
+// out[line] = max_k |a[line ld + k]| * (scale ? scale[k] : 1); one wave per line.
+__global__ void line_absmax_kernel(int64_t n, const double* __restrict__ a, int64_t lda,
+                                   const double* __restrict__ scale, double* __restrict__ out) {
+  const int64_t line = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (line >= n) return;
+  const double* p = a + line * lda;
   double acc = 0.0;
-  if (layout == kColMajor) {   // fast = i, slow = j
-    const int64_t i = fast;
-    for (int64_t j = s0; j < s1; ++j) {
-      double v = fabs(a[i + j * lda]);
-      if (rscale) {
-        v *= rscale[i];
-        if (v > cmax[j]) atomic_max_nonneg(&cmax[j], v);
-      } else {
-        acc = fmax(acc, v);
-      }
-    }
-    if (!rscale) atomic_max_nonneg(&rmax[i], acc);
-  } else {                     // fast = j, slow = i
-    const int64_t j = fast;
-    for (int64_t i = s0; i < s1; ++i) {
-      double v = fabs(a[i * lda + j]);
-      if (rscale) acc = fmax(acc, v * rscale[i]);
-      else if (v > rmax[i]) atomic_max_nonneg(&rmax[i], v);
-    }
-    if (rscale) atomic_max_nonneg(&cmax[j], acc);
-  }
+  for (int64_t k = lane; k < n; k += 64) acc = fmax(acc, scale ? fabs(p[k]) * scale[k] : fabs(p[k]));
+  for (int off = 32; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off));
+  if (lane == 0) out[line] = acc;
+}
+
+// out[k] = max over lines of |a[line ld + k]| * (scale ? scale[line] : 1): one
+// thread per position over a chunk of lines (blockIdx.y), one atomic per thread.
+__global__ void cross_absmax_kernel(int64_t n, const double* __restrict__ a, int64_t lda,
+                                    const double* __restrict__ scale, double* __restrict__ out,
+                                    int64_t chunk) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t l0 = blockIdx.y * chunk, l1 = std::min<int64_t>(n, l0 + chunk);
+  double acc = 0.0;
+  for (int64_t l = l0; l < l1; ++l) acc = fmax(acc, scale ? fabs(a[l * lda + k]) * scale[l] : fabs(a[l * lda + k]));
+  atomic_max_nonneg(&out[k], acc);
 }
 
 __global__ void to_scale_kernel(int64_t n, const double* __restrict__ amax, double* __restrict__ scale) {
@@ -219,47 +215,282 @@ __global__ void transpose_f32_kernel(int64_t kb, const float* __restrict__ src, 
   dst[i * kb + j] = (lower_only && j > i) ? 0.0f : src[j * lds + i];
 }
 
-// ---- blocked triangular solve helpers (vectors FP32) ---------------------------------
-// t[r] = y[r] - sum_{c in [c0, c1)} W[(b0 + r) ld + c] z[c]  for r < nb  (one wave per row)
-__global__ void rowdot_sub_kernel(int nb, const float* __restrict__ W, int64_t ld, int64_t b0,
-                                  int64_t c0, int64_t c1, const float* __restrict__ z,
-                                  const float* __restrict__ y, float* __restrict__ t) {
-  const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (r >= nb) return;
-  const float* w = W + (b0 + r) * ld;
-  double acc = 0.0;
-  for (int64_t c = c0 + lane; c < c1; c += 64) acc += (double)w[c] * (double)z[c];
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-  if (lane == 0) t[r] = (float)((double)y[r] - acc);
+// ---- optimistic panel: pivots confined to the diagonal block ----------------------------
+// Partial-pivoting LU of a 256 x 256 FP32 block held in the registers of one
+// 1024-thread workgroup: thread (ty, tx) of a 32 x 32 grid holds rows
+// ty + 32 i and columns tx + 32 j (i, j < 8). Row interchanges are virtual: a
+// chosen pivot row is marked done and the elimination simply skips done rows;
+// the block is written back in pivot order at the end, and ipiv gets the
+// LAPACK form (sequential interchanges, 1-based, offset by `base`). Pivot =
+// largest |a| among the rows not yet chosen, ties to the lowest row of the
+// block. l = a * fl(1/p) when |p| >= FLT_MIN, else a / p (the rule of zml's
+// getrf2.zig:65-87). A zero or NaN pivot sets *fail.
+constexpr int DB = 256;
+
+__device__ __forceinline__ uint64_t pivot_key(float v, int r) {
+  // larger |a| first, then lower row; NaN below every number, 0 = no candidate
+  if (v != v) return (uint64_t)(0x7fffffffu - (uint32_t)r);
+  return ((uint64_t)__float_as_uint(v) << 32) | (0xffffffffu - (uint32_t)r);
+}
+__device__ __forceinline__ int key_row(uint64_t k) {
+  const uint32_t lo = (uint32_t)k;
+  return lo >= 0x80000000u ? (int)(0xffffffffu - lo) : (int)(0x7fffffffu - lo);
 }
 
-// out[r] = sum_c M[r][c] t[c] (trans: M[c][r]) for an nb x nb block.
-__global__ void block_matvec_kernel(int nb, const float* __restrict__ M, int trans,
-                                    const float* __restrict__ t, float* __restrict__ out) {
-  const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (r >= nb) return;
-  double acc = 0.0;
-  for (int c = lane; c < nb; c += 64) acc += (double)(trans ? M[c * nb + r] : M[r * nb + c]) * (double)t[c];
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-  if (lane == 0) out[r] = (float)acc;
+__global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P, int64_t ld,
+                                                          int32_t* __restrict__ ipiv, int base,
+                                                          int* __restrict__ fail) {
+  __shared__ uint64_t wbest[16];
+  __shared__ float lcol[DB], urow[DB];
+  __shared__ int prow[DB], invp[DB], at[DB], pos[DB];
+  __shared__ int bad;
+  const int t = threadIdx.x, tx = t & 31, ty = t >> 5, lane = t & 63, wv = t >> 6;
+  float a[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[i][j] = P[(int64_t)(ty + 32 * i) * ld + tx + 32 * j];
+  if (t == 0) bad = 0;
+  unsigned done = 0;
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    for (int kq = 0; kq < 32; ++kq) {
+      const int k = 32 * jj + kq;
+      // candidates of column k: its owners (tx == kq), rows not yet chosen
+      uint64_t best = 0;
+      if (tx == kq) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (!((done >> i) & 1u)) {
+            const uint64_t key = pivot_key(fabsf(a[i][jj]), ty + 32 * i);
+            best = key > best ? key : best;
+          }
+      }
+      const uint64_t other = __shfl_xor(best, 32);
+      best = other > best ? other : best;
+      if (lane == kq) wbest[wv] = best;
+      __syncthreads();
+      uint64_t g = wbest[0];
+#pragma unroll
+      for (int w = 1; w < 16; ++w) g = wbest[w] > g ? wbest[w] : g;
+      const int p = key_row(g), pt = p & 31, pi = p >> 5;
+      if (ty == pt) {   // the pivot row's owners publish the row (its value at k is the pivot)
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (i == pi) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) urow[tx + 32 * j] = a[i][j];
+          }
+        done |= 1u << pi;
+      }
+      if (t == 0) prow[k] = p;
+      __syncthreads();
+      const float pv = urow[k];
+      const bool ok = pv != 0.0f && pv == pv;
+      if (!ok && t == 0) bad = 1;
+      const bool recip = fabsf(pv) >= 0x1p-126f;
+      const float rp = 1.0f / pv;
+      if (tx == kq) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          float l = 0.0f;
+          if (!((done >> i) & 1u) && ok) {
+            l = recip ? a[i][jj] * rp : a[i][jj] / pv;
+            a[i][jj] = l;
+          }
+          lcol[ty + 32 * i] = l;
+        }
+      }
+      __syncthreads();
+      float li[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) li[i] = lcol[ty + 32 * i];
+#pragma unroll
+      for (int j = jj; j < 8; ++j) {
+        const float u = urow[tx + 32 * j];
+        if (j > jj || tx > kq) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) a[i][j] -= li[i] * u;
+        }
+      }
+      // lcol and urow are rewritten after the next step's first barrier, wbest
+      // after its reads here: no further barrier is needed
+    }
+  }
+  // row r of the block goes to position invp[r]; ipiv by replaying the interchanges
+  if (t < DB) {
+    invp[prow[t]] = t;
+    at[t] = t;
+    pos[t] = t;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int k = 0; k < DB; ++k) {
+      const int p = prow[k], q = pos[p], r = at[k];
+      ipiv[k] = base + q + 1;
+      at[q] = r;
+      pos[r] = q;
+      at[k] = p;
+      pos[p] = k;
+    }
+    if (bad) atomicOr(fail, 1);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int64_t row = invp[ty + 32 * i];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) P[row * ld + tx + 32 * j] = a[i][j];
+  }
 }
 
-// y[i] = fl32(scale_i * v[perm[i]])
-__global__ void gather_scale_kernel(int64_t n, const double* __restrict__ v, const int32_t* __restrict__ perm,
-                                    const double* __restrict__ scale, float* __restrict__ y) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) y[i] = (float)(scale[perm[i]] * v[perm[i]]);
+// *fail |= 2 when some |x| of the rows x cols block is not <= tau (NaN included)
+__global__ void bound_check_kernel(int64_t rows, int cols, const float* __restrict__ X, int64_t ld,
+                                   float tau, int* __restrict__ fail) {
+  bool over = false;
+  const int64_t total = rows * cols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float v = X[(e / cols) * ld + e % cols];
+    over |= !(fabsf(v) <= tau);
+  }
+  if (__ballot(over) && (threadIdx.x & 63) == 0) atomicOr(fail, 2);
 }
 
-// out[i] = c_i w_i (mode 0) or out[i] += c_i w_i (mode 1), FP64
-__global__ void scale_out_kernel(int64_t n, const float* __restrict__ w, const double* __restrict__ c,
-                                 double* __restrict__ out, int accumulate) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double v = c[i] * (double)w[i];
-  out[i] = accumulate ? out[i] + v : v;
+// ---- triangular solves with the FP32 factors -----------------------------------------
+// One persistent launch per triangle. The factor is cut into TS x TS blocks;
+// workgroup g owns block rows g, g + G, ... (forward: in increasing order,
+// backward: decreasing). For block row b it accumulates
+//   t_b = y_b - sum_j T_bj z_j      (j < b forward, j > b backward)
+// taking each z_j as soon as its owner has published it, then
+//   z_b = D_b t_b,  D_b = inverse of the diagonal block T_bb,
+// and publishes z_b. Published values are 64-bit words (tag << 32 | float
+// bits) written and read with device-scope relaxed atomics: a reader knows a
+// value is current from its tag, so no cache-wide fence is needed. The grid
+// never exceeds the number of co-resident workgroups, and every block row
+// waits only on block rows earlier in its own order, so the waits always end;
+// a spin limit turns a stall into an error flag instead of a hang.
+// Layout of a workgroup: 4 waves x 32 rows; lane l covers columns l and l+64
+// of each column block, so every row read is two coalesced 256-byte loads.
+constexpr int TS = 128;
+constexpr int TS_RPW = 32;
+
+struct TriSolve {
+  int64_t n, nblk;
+  int upper;
+  const float* W;            // row-major factor, ld ldw
+  int64_t ldw;
+  const float* inv;          // inverses of the 256 x 256 diagonal blocks (row-major, ld = block size)
+  const double* v;           // forward: y_i = fl32(rsc[perm[i]] v[perm[i]]) ...
+  const int32_t* perm;
+  const double* rsc;
+  const float* yin;          // ... or y = yin when v is null
+  float* zout;               // plain copy of the solution (optional)
+  const double* csc;         // wout_i (+)= csc_i z_i (optional)
+  double* wout;
+  int accumulate;
+  uint64_t* xch;             // n tagged words
+  uint32_t tag;
+  int* err;
+  int64_t spin_limit;
+};
+
+__device__ __forceinline__ float wait_word(const uint64_t* p, uint32_t tag, int64_t limit, int* err) {
+  for (int64_t it = 0;; ++it) {
+    const uint64_t w = __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(w >> 32) == tag) return __uint_as_float((uint32_t)w);
+    if (it >= limit) {
+      atomicOr(err, 1);
+      return 0.0f;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+__global__ void __launch_bounds__(256) trisolve_f32_kernel(TriSolve a) {
+  __shared__ float red[4][TS_RPW][65];
+  __shared__ float tvec[TS];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int64_t bi = blockIdx.x; bi < a.nblk; bi += gridDim.x) {
+    const int64_t b = a.upper ? a.nblk - 1 - bi : bi;
+    const int64_t r0 = b * TS;
+    const int rows = (int)std::min<int64_t>(TS, a.n - r0);
+    const int wr0 = w * TS_RPW;   // first row of this wave within the block
+    float acc[TS_RPW];
+#pragma unroll
+    for (int rr = 0; rr < TS_RPW; ++rr) acc[rr] = 0.0f;
+    const int64_t cnt = a.upper ? a.nblk - 1 - b : b;
+    for (int64_t q = 0; q < cnt; ++q) {
+      const int64_t j = a.upper ? a.nblk - 1 - q : q;
+      const int64_t c0 = j * TS;
+      const int cols = (int)std::min<int64_t>(TS, a.n - c0);
+      // the factor's values first (independent of z_j), then wait for z_j
+      float l0[TS_RPW], l1[TS_RPW];
+#pragma unroll
+      for (int rr = 0; rr < TS_RPW; ++rr) {
+        const bool rok = wr0 + rr < rows;
+        const float* row = a.W + (r0 + wr0 + rr) * a.ldw + c0;
+        l0[rr] = rok && lane < cols ? row[lane] : 0.0f;
+        l1[rr] = rok && lane + 64 < cols ? row[lane + 64] : 0.0f;
+      }
+      const float z0 = lane < cols ? wait_word(a.xch + c0 + lane, a.tag, a.spin_limit, a.err) : 0.0f;
+      const float z1 = lane + 64 < cols ? wait_word(a.xch + c0 + lane + 64, a.tag, a.spin_limit, a.err) : 0.0f;
+#pragma unroll
+      for (int rr = 0; rr < TS_RPW; ++rr) acc[rr] = fmaf(l1[rr], z1, fmaf(l0[rr], z0, acc[rr]));
+    }
+    // t = y - (row sums of the partials)
+#pragma unroll
+    for (int rr = 0; rr < TS_RPW; ++rr) red[w][rr][lane] = acc[rr];
+    __syncthreads();
+    if (lane < TS_RPW) {
+      const int r = wr0 + lane;
+      float s = 0.0f;
+      for (int l = 0; l < 64; ++l) s += red[w][lane][l];
+      float y = 0.0f;
+      if (r < rows) {
+        const int64_t i = r0 + r;
+        if (a.v) {
+          const int32_t p = a.perm[i];
+          y = (float)(a.rsc[p] * a.v[p]);
+        } else {
+          y = a.yin[i];
+        }
+      }
+      tvec[r] = r < rows ? y - s : 0.0f;
+    }
+    __syncthreads();
+    // z_b = D_b t: D_b is the (o, o) sub-block of the inverse of its 256-block
+    const int64_t B = r0 / 256;
+    const int64_t o = r0 - B * 256;
+    const int64_t ldi = std::min<int64_t>(256, a.n - B * 256);
+    const float* D = a.inv + B * 256 * 256 + o * ldi + o;
+    const float t0 = lane < rows ? tvec[lane] : 0.0f;
+    const float t1 = lane + 64 < rows ? tvec[lane + 64] : 0.0f;
+#pragma unroll
+    for (int rr = 0; rr < TS_RPW; ++rr) {
+      const bool rok = wr0 + rr < rows;
+      const float* row = D + (wr0 + rr) * ldi;
+      const float d0 = rok && lane < rows ? row[lane] : 0.0f;
+      const float d1 = rok && lane + 64 < rows ? row[lane + 64] : 0.0f;
+      acc[rr] = fmaf(d1, t1, d0 * t0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < TS_RPW; ++rr) red[w][rr][lane] = acc[rr];
+    __syncthreads();
+    if (lane < TS_RPW && wr0 + lane < rows) {
+      float z = 0.0f;
+      for (int l = 0; l < 64; ++l) z += red[w][lane][l];
+      const int64_t i = r0 + wr0 + lane;
+      __hip_atomic_store(a.xch + i, ((uint64_t)a.tag << 32) | __float_as_uint(z), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      if (a.zout) a.zout[i] = z;
+      if (a.wout) {
+        const double v = a.csc[i] * (double)z;
+        a.wout[i] = a.accumulate ? a.wout[i] + v : v;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ---- FP64 vector kernels (refinement / GMRES) --------------------------------------------
@@ -330,6 +561,15 @@ struct MixedCtx {
   float* cs = nullptr;       // cols
   float* inv_all = nullptr;  // L11^-1 of every block column, NB x NB each
   float* tmp = nullptr;      // inverse scratch
+  // optimistic panels
+  bool optimistic = true;
+  float tau = 16.0f;         // largest |l| accepted below the diagonal block
+  float* pbak = nullptr;     // copy of the block column (rows x DB)
+  float* uinv = nullptr;     // U11^-1 (DB x DB) and two scratch blocks
+  float* scr1 = nullptr;
+  float* scr2 = nullptr;
+  int* pfail = nullptr;
+  int panels = 0, panels_fast = 0;
 };
 thread_local MixedCtx tl_mix;
 
@@ -399,6 +639,51 @@ void tri_inverse_f32(int64_t kb, const float* L, int64_t ld, float* inv, int uni
   }
 }
 
+// Optimistic block column: partial pivoting inside the diagonal block, then
+// L21 = A21 U11^-1. With every |l_ij| <= 1 no row below the block would have
+// won a pivot search, so the result is the partial-pivoting one (up to
+// rounding and ties); with |l_ij| <= tau > 1 it is threshold pivoting, whose
+// growth per step is at most 1 + tau (the FP64 refinement then restores the
+// accuracy, and GMRES-IR or the FP64 factorization take over if it cannot).
+// Otherwise the block column is restored and the caller factors it with the
+// full pivot search.
+bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int32_t* ipiv,
+                   hipStream_t s) {
+  MixedCtx& mx = tl_mix;
+  const int64_t rows = m - j0;
+  if (!mx.optimistic || jb != DB || rows < DB) return false;
+  ++mx.panels;
+  float* P = A + j0 * lda + j0;
+  ZH_CHECK(hipMemsetAsync(mx.pfail, 0, sizeof(int), s));
+  ZH_CHECK(hipMemcpy2DAsync(mx.pbak, sizeof(float) * DB, P, sizeof(float) * lda, sizeof(float) * DB, rows,
+                            hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(diag_getf2_kernel, dim3(1), dim3(1024), 0, s, P, lda, ipiv + j0, (int)j0, mx.pfail);
+  ZH_LAUNCH_CHECK();
+  if (rows > DB) {
+    // U11^-1 = ((U11^T)^-1)^T, then L21 = A21 U11^-1 from the saved A21
+    hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, P, lda,
+                       mx.scr1, 1);
+    tri_inverse_f32(DB, mx.scr1, DB, mx.scr2, 0, s);
+    hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr2,
+                       (int64_t)DB, mx.uinv, 0);
+    mp_gemm_f32(rows - DB, DB, DB, 1.0f, mx.pbak + (int64_t)DB * DB, DB, mx.uinv, DB, 0.0f, nullptr, nullptr,
+                P + DB * lda, lda, s);
+    hipLaunchKernelGGL(bound_check_kernel, dim3(blocks((rows - DB) * DB / 16)), dim3(256), 0, s, rows - DB, DB,
+                       (const float*)(P + DB * lda), lda, mx.tau, mx.pfail);
+    ZH_LAUNCH_CHECK();
+  }
+  int fail = 0;
+  ZH_CHECK(hipMemcpyAsync(&fail, mx.pfail, sizeof(int), hipMemcpyDeviceToHost, s));
+  ZH_CHECK(hipStreamSynchronize(s));
+  if (fail == 0) {
+    ++mx.panels_fast;
+    return true;
+  }
+  ZH_CHECK(hipMemcpy2DAsync(P, sizeof(float) * lda, mx.pbak, sizeof(float) * DB, sizeof(float) * DB, rows,
+                            hipMemcpyDeviceToDevice, s));
+  return false;
+}
+
 void invert_f32(int64_t kb, const float* L11, int64_t ld, float* inv, hipStream_t s) {
   tri_inverse_f32(kb, L11, ld, inv, 1, s);
 }
@@ -426,10 +711,11 @@ struct Solver {
   double* r = nullptr;         // row scales
   double* c = nullptr;         // column scales
   double* rowsum = nullptr;
-  float* y = nullptr;          // solve work (n each)
-  float* z = nullptr;
-  float* t = nullptr;
-  float* t2 = nullptr;
+  float* z = nullptr;          // forward-solve result (n)
+  uint64_t* xch = nullptr;     // tagged exchange words of the triangular solves (n)
+  int* serr = nullptr;         // triangular-solve stall flag
+  mutable uint32_t tag = 0;    // last tag used in xch
+  unsigned ts_grid = 1;        // workgroups of a triangular solve
   double* part = nullptr;      // reduction partials
   double anorm = 0.0;
   int64_t nblk = 0;
@@ -449,27 +735,34 @@ double reduce_host(const Solver& S, int64_t n, const double* x, const double* y,
 
 // w (FP64) = Dc U^-1 L^-1 P Dr v; accumulate: w += ... instead
 void lu_solve(const Solver& S, const double* v, double* w, int accumulate) {
-  const int64_t n = S.n;
-  hipLaunchKernelGGL(gather_scale_kernel, dim3(blocks(n)), dim3(256), 0, S.s, n, v, S.perm, S.r, S.y);
+  TriSolve f{};
+  f.n = S.n;
+  f.nblk = (S.n + TS - 1) / TS;
+  f.upper = 0;
+  f.W = S.W;
+  f.ldw = S.n;
+  f.inv = S.invL;
+  f.v = v;
+  f.perm = S.perm;
+  f.rsc = S.r;
+  f.zout = S.z;
+  f.xch = S.xch;
+  f.tag = ++S.tag;
+  f.err = S.serr;
+  f.spin_limit = int64_t(1) << 22;
+  hipLaunchKernelGGL(trisolve_f32_kernel, dim3(S.ts_grid), dim3(256), 0, S.s, f);
   ZH_LAUNCH_CHECK();
-  // forward: z_b = invL_b (y_b - L[b, :b] z[:b])
-  for (int64_t b = 0; b < S.nblk; ++b) {
-    const int64_t b0 = b * NB_MIX, nb = std::min<int64_t>(NB_MIX, n - b0);
-    hipLaunchKernelGGL(rowdot_sub_kernel, dim3(blocks(nb, 4)), dim3(256), 0, S.s, (int)nb, S.W, n, b0,
-                       (int64_t)0, b0, S.z, S.y + b0, S.t);
-    hipLaunchKernelGGL(block_matvec_kernel, dim3(blocks(nb, 4)), dim3(256), 0, S.s, (int)nb,
-                       S.invL + b * NB_MIX * NB_MIX, 0, S.t, S.z + b0);
-  }
-  // backward: y_b = invU_b (z_b - U[b, b+1:] y[b+1:])
-  for (int64_t b = S.nblk - 1; b >= 0; --b) {
-    const int64_t b0 = b * NB_MIX, nb = std::min<int64_t>(NB_MIX, n - b0);
-    hipLaunchKernelGGL(rowdot_sub_kernel, dim3(blocks(nb, 4)), dim3(256), 0, S.s, (int)nb, S.W, n, b0,
-                       b0 + nb, n, S.y, S.z + b0, S.t);
-    hipLaunchKernelGGL(block_matvec_kernel, dim3(blocks(nb, 4)), dim3(256), 0, S.s, (int)nb,
-                       S.invU + b * NB_MIX * NB_MIX, 1, S.t, S.y + b0);
-  }
-  ZH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(scale_out_kernel, dim3(blocks(n)), dim3(256), 0, S.s, n, S.y, S.c, w, accumulate);
+  TriSolve g = f;
+  g.upper = 1;
+  g.inv = S.invU;
+  g.v = nullptr;
+  g.yin = S.z;
+  g.zout = nullptr;
+  g.csc = S.c;
+  g.wout = w;
+  g.accumulate = accumulate;
+  g.tag = ++S.tag;
+  hipLaunchKernelGGL(trisolve_f32_kernel, dim3(S.ts_grid), dim3(256), 0, S.s, g);
   ZH_LAUNCH_CHECK();
 }
 
@@ -480,7 +773,15 @@ void residual(const Solver& S, const double* b, const double* x, double* r) {
   gemv_f64_cm(trans, S.n, S.n, -1.0, S.A, S.lda, x, 1, 1.0, r, 1, S.s);
 }
 
+void check_solve(const Solver& S) {
+  int e = 0;
+  ZH_CHECK(hipMemcpyAsync(&e, S.serr, sizeof(int), hipMemcpyDeviceToHost, S.s));
+  ZH_CHECK(hipStreamSynchronize(S.s));
+  if (e) throw HipError(hipErrorLaunchTimeOut, "triangular solve stalled (spin limit reached)");
+}
+
 double backward_error(const Solver& S, const double* r, const double* x) {
+  check_solve(S);
   const double rn = reduce_host(S, S.n, r, nullptr, 1);
   const double xn = reduce_host(S, S.n, x, nullptr, 1);
   return rn / (S.anorm * xn * (double)S.n * 0x1p-53);
@@ -592,7 +893,8 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   S.W = static_cast<float*>(workspace(sizeof(float) * (size_t)n * n, 8));
   char* small = static_cast<char*>(workspace(
       sizeof(float) * 2 * (size_t)S.nblk * NB_MIX * NB_MIX + sizeof(int32_t) * 2 * n + 64 +
-          sizeof(double) * 3 * n + sizeof(float) * 4 * (n + NB_MIX) + sizeof(double) * 512 + 4096, 9));
+          sizeof(double) * 3 * n + sizeof(float) * (n + NB_MIX) + sizeof(uint64_t) * (n + NB_MIX) + 64 +
+          sizeof(double) * 512 + 4096, 9));
   S.invL = reinterpret_cast<float*>(small); small += sizeof(float) * S.nblk * NB_MIX * NB_MIX;
   S.invU = reinterpret_cast<float*>(small); small += sizeof(float) * S.nblk * NB_MIX * NB_MIX;
   S.ipiv = reinterpret_cast<int32_t*>(small); small += sizeof(int32_t) * n;
@@ -601,11 +903,20 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   S.r = reinterpret_cast<double*>(small); small += sizeof(double) * n;
   S.c = reinterpret_cast<double*>(small); small += sizeof(double) * n;
   S.rowsum = reinterpret_cast<double*>(small); small += sizeof(double) * n;
-  S.y = reinterpret_cast<float*>(small); small += sizeof(float) * (n + NB_MIX);
   S.z = reinterpret_cast<float*>(small); small += sizeof(float) * (n + NB_MIX);
-  S.t = reinterpret_cast<float*>(small); small += sizeof(float) * (n + NB_MIX);
-  S.t2 = reinterpret_cast<float*>(small); small += sizeof(float) * (n + NB_MIX);
+  S.xch = reinterpret_cast<uint64_t*>(small); small += sizeof(uint64_t) * (n + NB_MIX);
+  S.serr = reinterpret_cast<int*>(small); small += 64;
   S.part = reinterpret_cast<double*>(small);
+  // tags start again from 1 on cleared exchange words
+  ZH_CHECK(hipMemsetAsync(S.xch, 0, sizeof(uint64_t) * (n + NB_MIX) + 64, s));
+  {
+    int dev = 0, cus = 0, per_cu = 0;
+    ZH_CHECK(hipGetDevice(&dev));
+    ZH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    ZH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trisolve_f32_kernel, 256, 0));
+    const int64_t nts = (n + TS - 1) / TS;
+    S.ts_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nts, (int64_t)cus * std::max(per_cu, 1)));
+  }
   const int64_t kp = NB_MIX;
   char* lp = static_cast<char*>(workspace(sizeof(_Float16) * 2 * (size_t)(n + NB_MIX) * kp +
                                               sizeof(float) * 2 * (n + NB_MIX) + 4096, 10));
@@ -616,15 +927,24 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   tl_mix.cs = reinterpret_cast<float*>(lp);
 
   // ---- 1. equilibration and the FP32 working matrix
+  // r_i = 2^-floor(log2 max_j |a_ij|), then c_j = 2^-floor(log2 max_i |r_i a_ij|)
   ZH_CHECK(hipMemsetAsync(S.r, 0, sizeof(double) * 2 * n, s));   // r, c hold the maxima first
   double* rmax = S.r;
   double* cmax = S.c;
-  const int64_t split = 64, chunk = (n + split - 1) / split;
-  hipLaunchKernelGGL(absmax_kernel, dim3(blocks(n), (unsigned)split), dim3(256), 0, s, layout, n, A, lda,
-                     (const double*)nullptr, rmax, cmax, chunk);
+  const int64_t split = std::max<int64_t>(1, std::min<int64_t>(256, n / 64)), chunk = (n + split - 1) / split;
+  if (layout == kRowMajor)
+    hipLaunchKernelGGL(line_absmax_kernel, dim3(blocks(n, 4)), dim3(256), 0, s, n, A, lda,
+                       (const double*)nullptr, rmax);
+  else
+    hipLaunchKernelGGL(cross_absmax_kernel, dim3(blocks(n), (unsigned)split), dim3(256), 0, s, n, A, lda,
+                       (const double*)nullptr, rmax, chunk);
   hipLaunchKernelGGL(to_scale_kernel, dim3(blocks(n)), dim3(256), 0, s, n, rmax, S.rowsum);   // rowsum <- r
-  hipLaunchKernelGGL(absmax_kernel, dim3(blocks(n), (unsigned)split), dim3(256), 0, s, layout, n, A, lda,
-                     (const double*)S.rowsum, rmax, cmax, chunk);
+  if (layout == kRowMajor)
+    hipLaunchKernelGGL(cross_absmax_kernel, dim3(blocks(n), (unsigned)split), dim3(256), 0, s, n, A, lda,
+                       (const double*)S.rowsum, cmax, chunk);
+  else
+    hipLaunchKernelGGL(line_absmax_kernel, dim3(blocks(n, 4)), dim3(256), 0, s, n, A, lda,
+                       (const double*)S.rowsum, cmax);
   ZH_CHECK(hipMemcpyAsync(S.r, S.rowsum, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(to_scale_kernel, dim3(blocks(n)), dim3(256), 0, s, n, cmax, S.c);
   ZH_LAUNCH_CHECK();
@@ -644,21 +964,38 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
       invert_f32(kb, L11, ld, inv, st);
     }
   };
-  const LuOps<float> ops{trailing_gemm, trsm_f32, Inv::store, apply_inv_f32, panel_gemm_f32};
+  {
+    char* pw = static_cast<char*>(workspace(sizeof(float) * ((size_t)(n + DB) * DB + 3 * DB * DB) + 256, 15));
+    tl_mix.pbak = reinterpret_cast<float*>(pw); pw += sizeof(float) * (size_t)(n + DB) * DB;
+    tl_mix.uinv = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
+    tl_mix.scr1 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
+    tl_mix.scr2 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
+    tl_mix.pfail = reinterpret_cast<int*>(pw);
+    tl_mix.panels = tl_mix.panels_fast = 0;
+    tl_mix.optimistic = opts.no_optimistic == 0;
+    tl_mix.tau = opts.pivot_threshold > 0.0 ? (float)opts.pivot_threshold : 16.0f;
+  }
+  const LuOps<float> ops{trailing_gemm, trsm_f32, Inv::store, apply_inv_f32, panel_gemm_f32, try_panel_f32};
   getrf_lookahead<float>(n, n, S.W, n, S.ipiv, S.info, NB_MIX, ops, s);
   int info = 0;
   ZH_CHECK(hipMemcpyAsync(&info, S.info, sizeof(int), hipMemcpyDeviceToHost, s));
   ZH_CHECK(hipStreamSynchronize(s));
   res->info = info;
+  res->panels = tl_mix.panels;
+  res->panels_optimistic = tl_mix.panels_fast;
 
-  // inverses of the diagonal blocks of L and of U^T (for the triangular solves)
-  float* scratch = static_cast<float*>(workspace(sizeof(float) * NB_MIX * NB_MIX, 11));
+  // inverses of the diagonal blocks of L and of U, row-major (for the triangular solves)
+  float* scratch = static_cast<float*>(workspace(sizeof(float) * 2 * NB_MIX * NB_MIX, 11));
+  float* scratch2 = scratch + NB_MIX * NB_MIX;
   for (int64_t b = 0; b < S.nblk; ++b) {
     const int64_t b0 = b * NB_MIX, nb = std::min<int64_t>(NB_MIX, n - b0);
     tri_inverse_f32(nb, S.W + b0 * n + b0, n, S.invL + b * NB_MIX * NB_MIX, 1, s);
+    // inv(U) = (inv(U^T))^T, U^T being lower
     hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(nb * nb)), dim3(256), 0, s, nb, S.W + b0 * n + b0, n,
                        scratch, 1);
-    tri_inverse_f32(nb, scratch, nb, S.invU + b * NB_MIX * NB_MIX, 0, s);
+    tri_inverse_f32(nb, scratch, nb, scratch2, 0, s);
+    hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(nb * nb)), dim3(256), 0, s, nb, scratch2, nb,
+                       S.invU + b * NB_MIX * NB_MIX, 0);
   }
   ZH_LAUNCH_CHECK();
   // row permutation: perm[i] = original row of factored row i

@@ -37,6 +37,11 @@ struct LuOps {
   // update inside a block column (C -= L U); the trailing update when null
   void (*panel_gemm)(int64_t rows, int64_t cols, int64_t kb, const T* L, int64_t ldl, const T* U,
                      int64_t ldu, T* C, int64_t ldc, hipStream_t s);
+  // optional: factor block column [j0, j0 + jb) (rows j0..m-1) with its pivots
+  // taken inside the diagonal block; returns false, with the block column
+  // unchanged, when that would not be the partial-pivoting result
+  bool (*try_panel)(int64_t m, T* A, int64_t lda, int64_t j0, int64_t jb, int32_t* ipiv,
+                    hipStream_t s);
 };
 
 int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,

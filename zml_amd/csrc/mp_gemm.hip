@@ -38,8 +38,27 @@ __device__ __forceinline__ void tile_of(int64_t bid, int64_t ntm, int64_t ntn, i
   tn = r / gm;
 }
 
-__device__ __forceinline__ void epilogue(const f16v (&acc)[2][2], int64_t m, int64_t n, float alpha,
-                                         float beta, const float* __restrict__ rs,
+// C is read before the K loop (its HBM latency then overlaps the MFMA work:
+// for a rank-256 update the C traffic, not the arithmetic, bounds the kernel)
+// and combined with the accumulators at the end.
+__device__ __forceinline__ void load_c(f16v (&cv)[2][2], int64_t m, int64_t n, float beta,
+                                       const float* __restrict__ C, int64_t ldc, int64_t r0, int64_t c0,
+                                       int lane) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int64_t j = c0 + 32 * bj + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = r0 + 32 * bi + 8 * (r >> 2) + 4 * ((lane >> 5) & 1) + (r & 3);
+        cv[bi][bj][r] = (beta != 0.0f && i < m && j < n) ? C[i * ldc + j] : 0.0f;
+      }
+    }
+}
+
+__device__ __forceinline__ void epilogue(const f16v (&acc)[2][2], const f16v (&cv)[2][2], int64_t m,
+                                         int64_t n, float alpha, float beta, const float* __restrict__ rs,
                                          const float* __restrict__ cs, float* __restrict__ C,
                                          int64_t ldc, int64_t r0, int64_t c0, int lane) {
 #pragma unroll
@@ -54,8 +73,7 @@ __device__ __forceinline__ void epilogue(const f16v (&acc)[2][2], int64_t m, int
         const int64_t i = r0 + 32 * bi + 8 * (r >> 2) + 4 * ((lane >> 5) & 1) + (r & 3);
         if (i >= m) continue;
         const float s = alpha * cj * (rs ? rs[i] : 1.0f);
-        float* p = C + i * ldc + j;
-        *p = beta == 0.0f ? s * acc[bi][bj][r] : beta * *p + s * acc[bi][bj][r];
+        C[i * ldc + j] = beta == 0.0f ? s * acc[bi][bj][r] : beta * cv[bi][bj][r] + s * acc[bi][bj][r];
       }
     }
 }
@@ -141,6 +159,8 @@ mp_gemm_h_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restri
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
 
+  f16v cv[2][2];
+  load_c(cv, m, n, beta, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
   const int nk = (int)((K + BKH - 1) / BKH);
   load(0);
   store(0);
@@ -166,7 +186,7 @@ mp_gemm_h_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restri
     if (kt + 1 < nk) store(cur ^ 1);
     __syncthreads();
   }
-  epilogue(acc, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
+  epilogue(acc, cv, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
 }
 
 // ---- FP32 ------------------------------------------------------------------------
@@ -223,6 +243,8 @@ mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __re
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
 
+  f16v cv[2][2];
+  load_c(cv, m, n, beta, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
   const int nk = (int)((K + BKF - 1) / BKF);
   load(0);
   store(0);
@@ -247,7 +269,7 @@ mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __re
     if (kt + 1 < nk) store(cur ^ 1);
     __syncthreads();
   }
-  epilogue(acc, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
+  epilogue(acc, cv, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
 }
 
 }  // namespace
