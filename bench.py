@@ -162,14 +162,16 @@ def main():
         if rc != 0:
             raise RuntimeError(f"FP64 path: {_lib.last_error()}")
 
-    fp64_step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.fp64_steps):
+    t_fp64 = be_fp64 = None
+    if args.fp64_steps > 0:
         fp64_step()
-    torch.cuda.synchronize()
-    t_fp64 = (time.perf_counter() - t0) / args.fp64_steps
-    be_fp64 = backward_error(x64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.fp64_steps):
+            fp64_step()
+        torch.cuda.synchronize()
+        t_fp64 = (time.perf_counter() - t0) / args.fp64_steps
+        be_fp64 = backward_error(x64)
     del Aw
 
     prec = args.precision
@@ -204,10 +206,11 @@ def main():
         "backward_error": round(be_mixed, 4),
         "error_vs_x_true": err_true,
         "ir_iterations": iters, "gmres_iterations": gm, "fallback": fb,
-        "plain_fp64": {"ms": round(t_fp64 * 1e3, 2), "tflops": round(flops / t_fp64 / 1e12, 3),
-                       "backward_error": round(be_fp64, 4),
-                       "frac_fp64_peak": round(flops / t_fp64 / 1e12 / PEAK["fp64"], 4)},
-        "speedup_vs_plain_fp64": round(t_fp64 / t_mixed, 3),
+        "plain_fp64": None if t_fp64 is None else {
+            "ms": round(t_fp64 * 1e3, 2), "tflops": round(flops / t_fp64 / 1e12, 3),
+            "backward_error": round(be_fp64, 4),
+            "frac_fp64_peak": round(flops / t_fp64 / 1e12 / PEAK["fp64"], 4)},
+        "speedup_vs_plain_fp64": None if t_fp64 is None else round(t_fp64 / t_mixed, 3),
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

@@ -165,7 +165,8 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
   auto panel_and_inverse = [&](int64_t j0, int64_t jb, int64_t blk) {
     if (!(ops.try_panel && ops.try_panel(m, A, lda, j0, jb, ipiv, sp)))
       factor_panel<T>(m, A, lda, j0, jb, ipiv, info, ops, sp, 2);
-    if (j0 + jb < n) ops.invert(jb, A + j0 * lda + j0, lda, inv + (blk & 1) * nb * nb, sp);
+    T* invp = ops.inv_store ? ops.inv_store(j0) : inv + (blk & 1) * nb * nb;
+    if (j0 + jb < n || ops.inv_store) ops.invert(jb, A + j0 * lda + j0, lda, invp, sp);
     ZH_CHECK(hipEventRecord(ev_panel, sp));
   };
   ZH_CHECK(hipEventRecord(ev_in, s));
@@ -173,7 +174,7 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
   panel_and_inverse(0, std::min(nb, mn), 0);
   for (int64_t j0 = 0, blk = 0; j0 < mn; j0 += nb, ++blk) {
     const int64_t jb = std::min(nb, mn - j0);
-    const T* invb = inv + (blk & 1) * nb * nb;
+    const T* invb = ops.inv_store ? ops.inv_store(j0) : inv + (blk & 1) * nb * nb;
     ZH_CHECK(hipStreamWaitEvent(s, ev_panel, 0));
     const int32_t* pr = compose_perm((int)j0, (int)(j0 + jb - 1), ipiv, s, 4);
     const int64_t c1 = j0 + jb;                       // first column right of the block
@@ -232,7 +233,8 @@ static void apply_inv_f64(int64_t kb, int64_t cols, const double* inv, double* B
 
 void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, int* info,
                   hipStream_t s) {
-  const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64, nullptr, nullptr};
+  const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64, nullptr, nullptr,
+                          nullptr};
   getrf_lookahead<double>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }
 

@@ -415,6 +415,20 @@ __global__ void __launch_bounds__(256) trisolve_f32_kernel(TriSolve a) {
     const int64_t r0 = b * TS;
     const int rows = (int)std::min<int64_t>(TS, a.n - r0);
     const int wr0 = w * TS_RPW;   // first row of this wave within the block
+    // D_b = the (o, o) sub-block of the inverse of its 256-block, fetched up
+    // front: only the last z_j is on the critical path, D_b must not be
+    const int64_t B = r0 / 256;
+    const int64_t o = r0 - B * 256;
+    const int64_t ldi = std::min<int64_t>(256, a.n - B * 256);
+    const float* D = a.inv + B * 256 * 256 + o * ldi + o;
+    float d0[TS_RPW], d1[TS_RPW];
+#pragma unroll
+    for (int rr = 0; rr < TS_RPW; ++rr) {
+      const bool rok = wr0 + rr < rows;
+      const float* row = D + (wr0 + rr) * ldi;
+      d0[rr] = rok && lane < rows ? row[lane] : 0.0f;
+      d1[rr] = rok && lane + 64 < rows ? row[lane + 64] : 0.0f;
+    }
     float acc[TS_RPW];
 #pragma unroll
     for (int rr = 0; rr < TS_RPW; ++rr) acc[rr] = 0.0f;
@@ -458,21 +472,11 @@ __global__ void __launch_bounds__(256) trisolve_f32_kernel(TriSolve a) {
       tvec[r] = r < rows ? y - s : 0.0f;
     }
     __syncthreads();
-    // z_b = D_b t: D_b is the (o, o) sub-block of the inverse of its 256-block
-    const int64_t B = r0 / 256;
-    const int64_t o = r0 - B * 256;
-    const int64_t ldi = std::min<int64_t>(256, a.n - B * 256);
-    const float* D = a.inv + B * 256 * 256 + o * ldi + o;
+    // z_b = D_b t
     const float t0 = lane < rows ? tvec[lane] : 0.0f;
     const float t1 = lane + 64 < rows ? tvec[lane + 64] : 0.0f;
 #pragma unroll
-    for (int rr = 0; rr < TS_RPW; ++rr) {
-      const bool rok = wr0 + rr < rows;
-      const float* row = D + (wr0 + rr) * ldi;
-      const float d0 = rok && lane < rows ? row[lane] : 0.0f;
-      const float d1 = rok && lane + 64 < rows ? row[lane + 64] : 0.0f;
-      acc[rr] = fmaf(d1, t1, d0 * t0);
-    }
+    for (int rr = 0; rr < TS_RPW; ++rr) acc[rr] = fmaf(d1[rr], t1, d0[rr] * t0);
     __syncthreads();
 #pragma unroll
     for (int rr = 0; rr < TS_RPW; ++rr) red[w][rr][lane] = acc[rr];
@@ -560,6 +564,8 @@ struct MixedCtx {
   float* rs = nullptr;       // rows
   float* cs = nullptr;       // cols
   float* inv_all = nullptr;  // L11^-1 of every block column, NB x NB each
+  float* invu_all = nullptr; // U11^-1 of every block column (filled by diagonal-block panels)
+  std::vector<char> have_invu;
   float* tmp = nullptr;      // inverse scratch
   // optimistic panels
   bool optimistic = true;
@@ -664,9 +670,10 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, P, lda,
                        mx.scr1, 1);
     tri_inverse_f32(DB, mx.scr1, DB, mx.scr2, 0, s);
+    float* uinv = mx.invu_all + (j0 / DB) * DB * DB;
     hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr2,
-                       (int64_t)DB, mx.uinv, 0);
-    mp_gemm_f32(rows - DB, DB, DB, 1.0f, mx.pbak + (int64_t)DB * DB, DB, mx.uinv, DB, 0.0f, nullptr, nullptr,
+                       (int64_t)DB, uinv, 0);
+    mp_gemm_f32(rows - DB, DB, DB, 1.0f, mx.pbak + (int64_t)DB * DB, DB, uinv, DB, 0.0f, nullptr, nullptr,
                 P + DB * lda, lda, s);
     hipLaunchKernelGGL(bound_check_kernel, dim3(blocks((rows - DB) * DB / 16)), dim3(256), 0, s, rows - DB, DB,
                        (const float*)(P + DB * lda), lda, mx.tau, mx.pfail);
@@ -677,6 +684,7 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
   ZH_CHECK(hipStreamSynchronize(s));
   if (fail == 0) {
     ++mx.panels_fast;
+    if (rows > DB) mx.have_invu[j0 / DB] = 1;
     return true;
   }
   ZH_CHECK(hipMemcpy2DAsync(P, sizeof(float) * lda, mx.pbak, sizeof(float) * DB, sizeof(float) * DB, rows,
@@ -972,10 +980,17 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
     tl_mix.scr2 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
     tl_mix.pfail = reinterpret_cast<int*>(pw);
     tl_mix.panels = tl_mix.panels_fast = 0;
+    tl_mix.inv_all = S.invL;
+    tl_mix.invu_all = S.invU;
+    tl_mix.have_invu.assign(S.nblk, 0);
     tl_mix.optimistic = opts.no_optimistic == 0;
     tl_mix.tau = opts.pivot_threshold > 0.0 ? (float)opts.pivot_threshold : 16.0f;
   }
-  const LuOps<float> ops{trailing_gemm, trsm_f32, Inv::store, apply_inv_f32, panel_gemm_f32, try_panel_f32};
+  struct InvAt {
+    static float* at(int64_t j0) { return tl_mix.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX; }
+  };
+  const LuOps<float> ops{trailing_gemm, trsm_f32, Inv::store, apply_inv_f32, panel_gemm_f32, try_panel_f32,
+                         InvAt::at};
   getrf_lookahead<float>(n, n, S.W, n, S.ipiv, S.info, NB_MIX, ops, s);
   int info = 0;
   ZH_CHECK(hipMemcpyAsync(&info, S.info, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -987,9 +1002,10 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   // inverses of the diagonal blocks of L and of U, row-major (for the triangular solves)
   float* scratch = static_cast<float*>(workspace(sizeof(float) * 2 * NB_MIX * NB_MIX, 11));
   float* scratch2 = scratch + NB_MIX * NB_MIX;
+  // (L11^-1 of every block and U11^-1 of the diagonal-block panels are already there)
   for (int64_t b = 0; b < S.nblk; ++b) {
     const int64_t b0 = b * NB_MIX, nb = std::min<int64_t>(NB_MIX, n - b0);
-    tri_inverse_f32(nb, S.W + b0 * n + b0, n, S.invL + b * NB_MIX * NB_MIX, 1, s);
+    if (tl_mix.have_invu[b]) continue;
     // inv(U) = (inv(U^T))^T, U^T being lower
     hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(nb * nb)), dim3(256), 0, s, nb, S.W + b0 * n + b0, n,
                        scratch, 1);

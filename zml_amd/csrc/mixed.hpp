@@ -42,6 +42,9 @@ struct LuOps {
   // unchanged, when that would not be the partial-pivoting result
   bool (*try_panel)(int64_t m, T* A, int64_t lda, int64_t j0, int64_t jb, int32_t* ipiv,
                     hipStream_t s);
+  // optional: where L11^-1 of the block column starting at j0 is kept (the
+  // driver's two alternating buffers when null)
+  T* (*inv_store)(int64_t j0);
 };
 
 int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,
