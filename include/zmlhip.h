@@ -116,6 +116,19 @@ int zmlhip_dgesv_mixed(int layout, int n, int nrhs, const double *a, int lda, co
                        int ldb, double *x, int ldx, const zmlhip_ir_opts *opts,
                        zmlhip_ir_result *result, void *stream);
 
+/* A x = b for symmetric positive definite A (n x n, device, layout 101/102,
+ * only the `uplo` ('L' or 'U') triangle is read): FP32 Cholesky whose
+ * diagonal blocks are factored in FP64 and whose trailing updates run in
+ * opts->precision, then FP64 refinement of the right-hand sides in groups of
+ * 16 (classic, GMRES-IR, FP64 Cholesky as in zmlhip_dgesv_mixed). result->info
+ * is the first non-positive pivot of the unshifted low-precision
+ * factorization (0 if none); a failure there is answered by factoring again
+ * with a small diagonal shift. Synchronous with the host. Returns 0, -i for an
+ * invalid argument, or a ZMLHIP_E* code. */
+int zmlhip_dposv_mixed(int layout, char uplo, int n, int nrhs, const double *a, int lda,
+                       const double *b, int ldb, double *x, int ldx, const zmlhip_ir_opts *opts,
+                       zmlhip_ir_result *result, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

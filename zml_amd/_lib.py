@@ -62,6 +62,8 @@ SIGNATURES = {
     "zmlhip_dpotrs": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _c_int, _p]),
     "zmlhip_dgesv_mixed": (_c_int, [_c_int, _c_int, _c_int, _p, _c_int, _p, _c_int, _p, _c_int, _p, _p,
                                     _p]),
+    "zmlhip_dposv_mixed": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _c_int, _p, _c_int,
+                                    _p, _p, _p]),
 }
 
 

@@ -405,6 +405,24 @@ int zmlhip_dgesv_mixed(int layout, int n, int nrhs, const double* a, int lda, co
   });
 }
 
+int zmlhip_dposv_mixed(int layout, char uplo, int n, int nrhs, const double* a, int lda, const double* b,
+                       int ldb, double* x, int ldx, const zmlhip_ir_opts* opts, zmlhip_ir_result* result,
+                       void* stream) {
+  return guarded([&] {
+    if (!valid_layout(layout)) return -1;
+    const int ul = uplo_from_char(uplo);
+    if (!ul) return -2;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (lda < std::max(1, n)) return -6;
+    if (ldb < std::max(1, layout == kColMajor ? n : nrhs)) return -8;
+    if (ldx < std::max(1, layout == kColMajor ? n : nrhs)) return -10;
+    if (opts && (opts->precision < 0 || opts->precision > 2)) return -11;
+    if (!result) return -12;
+    return dposv_mixed(layout, ul, n, nrhs, a, lda, b, ldb, x, ldx, opts, result, as_stream(stream));
+  });
+}
+
 // =============================================================================
 // Host-pointer CBLAS (include/cblas.h): synchronous, errors ignored as zml does
 // =============================================================================

@@ -727,6 +727,12 @@ struct Solver {
   double* part = nullptr;      // reduction partials
   double anorm = 0.0;
   int64_t nblk = 0;
+  // w (FP64) = M^-1 v (accumulate: w += M^-1 v), M the low-precision factorization
+  void (*precond)(const Solver&, const double* v, double* w, int accumulate) = nullptr;
+  // Cholesky only
+  float* invLT = nullptr;      // transposed diagonal-block inverses of L
+  float* ybuf = nullptr;       // (n + NB) x 16 right-hand sides, FP32 row-major
+  float* tbuf = nullptr;       // NB x 16
 };
 
 double reduce_host(const Solver& S, int64_t n, const double* x, const double* y, int mode) {
@@ -782,6 +788,7 @@ void residual(const Solver& S, const double* b, const double* x, double* r) {
 }
 
 void check_solve(const Solver& S) {
+  if (!S.serr) return;   // no persistent triangular solves in use
   int e = 0;
   ZH_CHECK(hipMemcpyAsync(&e, S.serr, sizeof(int), hipMemcpyDeviceToHost, S.s));
   ZH_CHECK(hipStreamSynchronize(S.s));
@@ -822,7 +829,7 @@ int gmres_ir(const Solver& S, const double* b, double* x, int restart, int max_i
     for (; j < m && total < max_it; ++j, ++total) {
       double* vj = V + (size_t)j * n;
       double* zj = Z + (size_t)j * n;
-      lu_solve(S, vj, zj, 0);
+      S.precond(S, vj, zj, 0);
       // w = A z_j
       const int trans = S.layout == kColMajor ? kNoTrans : kTrans;
       gemv_f64_cm(trans, n, n, 1.0, S.A, S.lda, zj, 1, 0.0, w, 1, S.s);
@@ -870,6 +877,269 @@ int gmres_ir(const Solver& S, const double* b, double* x, int restart, int max_i
   return total;
 }
 
+// ---- mixed-precision Cholesky ------------------------------------------------------------
+// S (n x n, FP64, row-major) = the symmetric matrix whose `uplo` triangle the
+// caller stores in a (layout, lda); 32 x 32 tiles of the lower triangle, each
+// written to (I, J) and, transposed through LDS, to (J, I).
+__global__ void symmetrize_kernel(int layout, int uplo, int64_t n, const double* __restrict__ a,
+                                  int64_t lda, double* __restrict__ S) {
+  __shared__ double tile[32][33];
+  const int64_t I = blockIdx.y * 32, J = blockIdx.x * 32;
+  if (J > I) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;   // 32 x 8
+  // element (i, j), i >= j, of the lower triangle, as stored by the caller
+  const bool rowwise = (layout == kRowMajor) == (uplo == kLower);   // j contiguous in memory
+  for (int k = ty; k < 32; k += 8) {
+    const int ii = rowwise ? k : tx, jj = rowwise ? tx : k;
+    const int64_t i = I + ii, j = J + jj;
+    double v = 0.0;
+    if (i < n && j < n && i >= j) {
+      if (layout == kRowMajor) v = uplo == kLower ? a[i * lda + j] : a[j * lda + i];
+      else v = uplo == kLower ? a[i + j * lda] : a[j + i * lda];
+    }
+    tile[ii][jj] = v;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t i = I + k, j = J + tx;   // S(i, j), i >= j
+    if (i < n && j < n && i >= j) S[i * n + j] = tile[k][tx];
+    const int64_t i2 = J + k, j2 = I + tx;   // S(i2, j2) = S(j2, i2), i2 <= j2
+    if (i2 < n && j2 < n && j2 > i2) S[i2 * n + j2] = tile[tx][k];
+  }
+}
+
+// d_i = 2^-floor(log2 sqrt(a_ii)) (1 for a non-positive or non-finite diagonal)
+__global__ void diag_scale_kernel(int64_t n, const double* __restrict__ S, double* __restrict__ d) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double v = S[i * n + i];
+    d[i] = v > 0.0 ? pow2_scale(sqrt(v)) : 1.0;
+  }
+}
+
+// dst (kb x kb, FP64, ld ldd) = src (FP32, ld lds) + shift I
+__global__ void block_to_f64_kernel(int64_t kb, const float* __restrict__ src, int64_t lds,
+                                    double* __restrict__ dst, int64_t ldd, double shift) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= kb * kb) return;
+  const int64_t i = e / kb, j = e % kb;
+  dst[i * ldd + j] = (double)src[i * lds + j] + (i == j ? shift : 0.0);
+}
+
+// dst[i][j] = fl32(trans ? src[j][i] : src[i][j]) for a kb x kb block; lower_only
+// writes j <= i only
+__global__ void block_to_f32_kernel(int64_t kb, const double* __restrict__ src, int64_t lds,
+                                    float* __restrict__ dst, int64_t ldd, int trans, int lower_only) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= kb * kb) return;
+  const int64_t i = e / kb, j = e % kb;
+  if (lower_only && j > i) return;
+  dst[i * ldd + j] = (float)(trans ? src[j * lds + i] : src[i * lds + j]);
+}
+
+__global__ void fill_identity_f64_kernel(int64_t kb, double* __restrict__ a) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e < kb * kb) a[e] = (e / kb == e % kb) ? 1.0 : 0.0;
+}
+
+// dst (cols x rows, ld ldd) = src^T (rows x cols, ld lds), 32 x 32 tiles
+__global__ void transpose_rect_f32_kernel(int64_t rows, int64_t cols, const float* __restrict__ src,
+                                          int64_t lds, float* __restrict__ dst, int64_t ldd) {
+  __shared__ float tile[32][33];
+  const int64_t R = blockIdx.y * 32, Cc = blockIdx.x * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t i = R + k, j = Cc + tx;
+    tile[k][tx] = (i < rows && j < cols) ? src[i * lds + j] : 0.0f;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t j = Cc + k, i = R + tx;
+    if (i < rows && j < cols) dst[j * ldd + i] = tile[tx][k];
+  }
+}
+
+// Y (n x nr, FP32 row-major) = fl32(d_i R_iq), R column-major (ld ldr)
+__global__ void pack_rows_kernel(int64_t n, int nr, const double* __restrict__ R, int64_t ldr,
+                                 const double* __restrict__ d, float* __restrict__ Y) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n * nr) return;
+  const int64_t i = e / nr;
+  const int q = (int)(e % nr);
+  Y[e] = (float)(d[i] * R[q * ldr + i]);
+}
+// D_iq (+)= d_i Y_iq
+__global__ void unpack_rows_kernel(int64_t n, int nr, const float* __restrict__ Y,
+                                   const double* __restrict__ d, double* __restrict__ D, int64_t ldd,
+                                   int accumulate) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n * nr) return;
+  const int64_t i = e / nr;
+  const int q = (int)(e % nr);
+  const double v = d[i] * (double)Y[e];
+  double* p = D + q * ldd + i;
+  *p = accumulate ? *p + v : v;
+}
+
+// D_iq += d_i Y_iq for the columns q set in mask
+__global__ void unpack_add_masked_kernel(int64_t n, int nr, const float* __restrict__ Y,
+                                         const double* __restrict__ d, double* __restrict__ D, int64_t ldd,
+                                         unsigned mask) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n * nr) return;
+  const int64_t i = e / nr;
+  const int q = (int)(e % nr);
+  if ((mask >> q) & 1u) D[q * ldd + i] += d[i] * (double)Y[e];
+}
+
+// C (k0 x nr, ld nr) -= A^T B, A (kb x k0, row-major, ld lda), B (kb x nr, ld nr), nr <= 16:
+// one thread per row of C, B staged in LDS
+constexpr int MAXR = 16;
+__global__ void __launch_bounds__(256) atb_sub_kernel(int64_t k0, int kb, int nr, const float* __restrict__ A,
+                                                      int64_t lda, const float* __restrict__ B,
+                                                      float* __restrict__ C) {
+  __shared__ float Bs[256 * MAXR];
+  for (int e = threadIdx.x; e < kb * nr; e += blockDim.x) Bs[e] = B[e];
+  __syncthreads();
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= k0) return;
+  float acc[MAXR];
+#pragma unroll
+  for (int q = 0; q < MAXR; ++q) acc[q] = 0.0f;
+  for (int r = 0; r < kb; ++r) {
+    const float av = A[r * lda + c];
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q)
+      if (q < nr) acc[q] = fmaf(av, Bs[r * nr + q], acc[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < MAXR; ++q)
+    if (q < nr) C[c * nr + q] -= acc[q];
+}
+
+// out[q] = max_i |M[q ld + i]| (out zeroed by the caller)
+__global__ void colmax_kernel(int64_t n, const double* __restrict__ M, int64_t ld, double* __restrict__ out) {
+  const int q = blockIdx.y;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc = fmax(acc, fabs(M[q * ld + i]));
+  for (int off = 32; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off));
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&out[q], acc);
+}
+
+struct CholWork {
+  double* d64 = nullptr;       // diagonal block, FP64
+  double* inv64 = nullptr;     // its inverse
+  int* info_blk = nullptr;     // potrf info of every diagonal block
+  float* tmp = nullptr;        // A21 copy, (n + NB) x NB
+  float* l21t = nullptr;       // L21^T for the FP32 update, NB x (n + NB)
+  _Float16* l16 = nullptr;     // 16-bit L21, (n + NB) x NB
+  float* rs = nullptr;         // its row scales
+};
+
+// Lower Cholesky of W + shift I (W row-major, the lower triangle): diagonal
+// blocks in FP64 (potrf and their inverses), L21 = A21 L11^-T as an FP32 GEMM
+// with that inverse, the trailing lower triangle updated in FP32 / FP16 / BF16.
+void chol_factor(Solver& S, const CholWork& cw, int prec, double shift) {
+  const int64_t n = S.n;
+  hipStream_t s = S.s;
+  ZH_CHECK(hipMemsetAsync(cw.info_blk, 0, sizeof(int) * S.nblk, s));
+  for (int64_t j0 = 0, b = 0; j0 < n; j0 += NB_MIX, ++b) {
+    const int64_t jb = std::min<int64_t>(NB_MIX, n - j0), m2 = n - j0 - jb;
+    float* Wd = S.W + j0 * n + j0;
+    hipLaunchKernelGGL(block_to_f64_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, (const float*)Wd, n,
+                       cw.d64, jb, shift);
+    ZH_LAUNCH_CHECK();
+    // row-major lower = column-major upper of the same memory
+    potrf_f64_cm(kUpper, jb, cw.d64, jb, cw.info_blk + b, s);
+    hipLaunchKernelGGL(fill_identity_f64_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, cw.inv64);
+    ZH_LAUNCH_CHECK();
+    // (row-major) L11 X = I  <=>  (column-major) X' U = I with U the column-major view of L11
+    trsm_f64_cm(kRight, kUpper, kNoTrans, kNonUnit, jb, jb, 1.0, cw.d64, jb, cw.inv64, jb, s);
+    float* invL = S.invL + b * NB_MIX * NB_MIX;
+    float* invLT = S.invLT + b * NB_MIX * NB_MIX;
+    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, (const double*)cw.d64, jb,
+                       Wd, n, 0, 1);
+    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb,
+                       (const double*)cw.inv64, jb, invL, jb, 0, 0);
+    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb,
+                       (const double*)cw.inv64, jb, invLT, jb, 1, 0);
+    ZH_LAUNCH_CHECK();
+    if (m2 <= 0) continue;
+    float* A21 = S.W + (j0 + jb) * n + j0;
+    ZH_CHECK(hipMemcpy2DAsync(cw.tmp, sizeof(float) * jb, A21, sizeof(float) * n, sizeof(float) * jb, m2,
+                              hipMemcpyDeviceToDevice, s));
+    mp_gemm_f32(m2, jb, jb, 1.0f, cw.tmp, jb, invLT, jb, 0.0f, nullptr, nullptr, A21, n, s);
+    float* C = S.W + (j0 + jb) * n + j0 + jb;
+    void* tok = profile_begin(kProfGemmLowp, s);
+    if (prec == kPrecF32) {
+      hipLaunchKernelGGL(transpose_rect_f32_kernel, dim3(blocks(jb, 32), blocks(m2, 32)), dim3(32, 8), 0, s, m2, jb,
+                         (const float*)A21, n, cw.l21t, m2);
+      ZH_LAUNCH_CHECK();
+      mp_gemm_f32(m2, m2, jb, -1.0f, A21, n, cw.l21t, m2, 1.0f, nullptr, nullptr, C, n, s, 1);
+    } else {
+      const int64_t kp = NB_MIX;
+      if (prec == kPrecF16) {
+        hipLaunchKernelGGL((rows_to_lowp_kernel<_Float16>), dim3(blocks(m2, 4)), dim3(256), 0, s, m2, (int)jb,
+                           (const float*)A21, n, cw.l16, kp, cw.rs);
+        ZH_LAUNCH_CHECK();
+        mp_gemm_f16(m2, m2, jb, -1.0f, cw.l16, kp, cw.l16, kp, 1.0f, cw.rs, cw.rs, C, n, s, 1);
+      } else {
+        __bf16* l = reinterpret_cast<__bf16*>(cw.l16);
+        hipLaunchKernelGGL((rows_to_lowp_kernel<__bf16>), dim3(blocks(m2, 4)), dim3(256), 0, s, m2, (int)jb,
+                           (const float*)A21, n, l, kp, cw.rs);
+        ZH_LAUNCH_CHECK();
+        mp_gemm_bf16(m2, m2, jb, -1.0f, l, kp, l, kp, 1.0f, cw.rs, cw.rs, C, n, s, 1);
+      }
+    }
+    // lower triangle only: m2 (m2 + 1) jb flops
+    profile_end(tok, s, (double)m2 * (m2 + 1) * jb, 4.0 * (double)m2 * (m2 + 1) + 2.0 * m2 * jb);
+  }
+}
+
+// Y (n x nr, FP32 row-major, ld nr) <- L^-T L^-1 Y with the blocked factor in S.W
+void chol_trsm_block(const Solver& S, int nr, float* Y) {
+  const int64_t n = S.n;
+  hipStream_t s = S.s;
+  for (int64_t k0 = 0, b = 0; k0 < n; k0 += NB_MIX, ++b) {   // L z = y
+    const int64_t kb = std::min<int64_t>(NB_MIX, n - k0);
+    mp_gemm_f32(kb, nr, kb, 1.0f, S.invL + b * NB_MIX * NB_MIX, kb, Y + k0 * nr, nr, 0.0f, nullptr, nullptr,
+                S.tbuf, nr, s);
+    ZH_CHECK(hipMemcpyAsync(Y + k0 * nr, S.tbuf, sizeof(float) * kb * nr, hipMemcpyDeviceToDevice, s));
+    if (k0 + kb < n)
+      mp_gemm_f32(n - k0 - kb, nr, kb, -1.0f, S.W + (k0 + kb) * n + k0, n, Y + k0 * nr, nr, 1.0f, nullptr,
+                  nullptr, Y + (k0 + kb) * nr, nr, s);
+  }
+  for (int64_t b = S.nblk - 1; b >= 0; --b) {                  // L^T x = z
+    const int64_t k0 = b * NB_MIX, kb = std::min<int64_t>(NB_MIX, n - k0);
+    mp_gemm_f32(kb, nr, kb, 1.0f, S.invLT + b * NB_MIX * NB_MIX, kb, Y + k0 * nr, nr, 0.0f, nullptr, nullptr,
+                S.tbuf, nr, s);
+    ZH_CHECK(hipMemcpyAsync(Y + k0 * nr, S.tbuf, sizeof(float) * kb * nr, hipMemcpyDeviceToDevice, s));
+    if (k0 > 0) {
+      hipLaunchKernelGGL(atb_sub_kernel, dim3(blocks(k0)), dim3(256), 0, s, k0, (int)kb, nr,
+                         (const float*)(S.W + k0 * n), n, (const float*)(Y + k0 * nr), Y);
+      ZH_LAUNCH_CHECK();
+    }
+  }
+}
+
+// D (n x nr, column-major, ld ldd) (+)= Dg L^-T L^-1 Dg R
+void chol_solve_block(const Solver& S, int nr, const double* R, int64_t ldr, double* D, int64_t ldd,
+                      int accumulate) {
+  const int64_t n = S.n;
+  hipLaunchKernelGGL(pack_rows_kernel, dim3(blocks(n * nr)), dim3(256), 0, S.s, n, nr, R, ldr,
+                     (const double*)S.r, S.ybuf);
+  ZH_LAUNCH_CHECK();
+  chol_trsm_block(S, nr, S.ybuf);
+  hipLaunchKernelGGL(unpack_rows_kernel, dim3(blocks(n * nr)), dim3(256), 0, S.s, n, nr, (const float*)S.ybuf,
+                     (const double*)S.r, D, ldd, accumulate);
+  ZH_LAUNCH_CHECK();
+}
+
+void chol_precond(const Solver& S, const double* v, double* w, int accumulate) {
+  chol_solve_block(S, 1, v, S.n, w, S.n, accumulate);
+}
+
 }  // namespace
 
 // Factor, solve and refine; see the file comment. x (FP64, ld ldx) receives the
@@ -897,6 +1167,7 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   S.lda = lda;
   S.s = s;
   S.nblk = (n + NB_MIX - 1) / NB_MIX;
+  S.precond = lu_solve;
   // ---- workspaces (slots 8..): the FP32 factors dominate (4 n^2 bytes)
   S.W = static_cast<float*>(workspace(sizeof(float) * (size_t)n * n, 8));
   char* small = static_cast<char*>(workspace(
@@ -1100,6 +1371,201 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   ZH_CHECK(hipStreamSynchronize(s));
   res->backward_error = worst;
   return info;
+}
+
+
+// Mixed-precision Cholesky solve of an SPD system (no counterpart in zml; the
+// FP64 interface it sits behind is dposv's, posv.zig:15-70):
+//   1. S = the full symmetric FP64 matrix from the caller's triangle (used by
+//      every residual), d_i = 2^-floor(log2 sqrt(s_ii)), W = fl32(D S D);
+//   2. W = L L^T (chol_factor). A non-positive pivot of the low-precision
+//      factorization is answered by factoring again with a diagonal shift
+//      (W + sigma I, sigma growing by 16 from 2^-10 on the unit-scale diagonal);
+//      the refinement then works with that preconditioner;
+//   3. the right-hand sides in groups of 16: X = D L^-T L^-1 D B, then FP64
+//      refinement of the whole group (R = B - S X as one FP64 GEMM) until
+//      every column meets the target; a column whose backward error stops
+//      halving, or grows, goes to GMRES-IR, and one that GMRES-IR cannot
+//      finish is solved with the FP64 Cholesky factorization of S.
+int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                const double* B, int64_t ldb, double* X, int64_t ldx, const zmlhip_ir_opts* o,
+                zmlhip_ir_result* res, hipStream_t s) {
+  zmlhip_ir_opts opts;
+  if (o) opts = *o;
+  else {
+    std::memset(&opts, 0, sizeof(opts));
+    opts.precision = kPrecF16;
+  }
+  if (opts.max_iter <= 0) opts.max_iter = 30;
+  if (opts.gmres_restart <= 0) opts.gmres_restart = 30;
+  if (opts.gmres_max_iter <= 0) opts.gmres_max_iter = 300;
+  if (!(opts.tol > 0.0)) opts.tol = 1.0;
+  std::memset(res, 0, sizeof(*res));
+  if (n == 0 || nrhs == 0) return 0;
+
+  Solver S;
+  S.layout = kRowMajor;   // S.A is the full symmetric copy
+  S.n = n;
+  S.s = s;
+  S.nblk = (n + NB_MIX - 1) / NB_MIX;
+  S.precond = chol_precond;
+  double* S64 = static_cast<double*>(workspace(sizeof(double) * (size_t)n * n, 16));
+  S.A = S64;
+  S.lda = n;
+  S.W = static_cast<float*>(workspace(sizeof(float) * (size_t)n * n, 8));
+  const size_t blk2 = (size_t)NB_MIX * NB_MIX;
+  char* small = static_cast<char*>(workspace(
+      sizeof(float) * 2 * S.nblk * blk2 + sizeof(double) * 2 * n + sizeof(double) * 512 +
+          sizeof(float) * ((n + NB_MIX) * MAXR + NB_MIX * MAXR) + sizeof(double) * 2 * blk2 +
+          sizeof(int) * (S.nblk + 64) + sizeof(double) * 2 * MAXR + 4096, 9));
+  auto carve = [&](size_t bytes) { char* p = small; small += (bytes + 255) / 256 * 256; return p; };
+  S.invL = reinterpret_cast<float*>(carve(sizeof(float) * S.nblk * blk2));
+  S.invLT = reinterpret_cast<float*>(carve(sizeof(float) * S.nblk * blk2));
+  S.r = reinterpret_cast<double*>(carve(sizeof(double) * n));
+  S.c = S.r;
+  S.rowsum = reinterpret_cast<double*>(carve(sizeof(double) * n));
+  S.part = reinterpret_cast<double*>(carve(sizeof(double) * 512));
+  S.ybuf = reinterpret_cast<float*>(carve(sizeof(float) * (n + NB_MIX) * MAXR));
+  S.tbuf = reinterpret_cast<float*>(carve(sizeof(float) * NB_MIX * MAXR));
+  CholWork cw;
+  cw.d64 = reinterpret_cast<double*>(carve(sizeof(double) * blk2));
+  cw.inv64 = reinterpret_cast<double*>(carve(sizeof(double) * blk2));
+  cw.info_blk = reinterpret_cast<int*>(carve(sizeof(int) * (S.nblk + 64)));
+  double* cmax = reinterpret_cast<double*>(carve(sizeof(double) * 2 * MAXR));
+  char* lp = static_cast<char*>(workspace(sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX * 2 +
+                                              sizeof(float) * (n + NB_MIX) + 4096, 10));
+  cw.tmp = reinterpret_cast<float*>(lp); lp += sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX;
+  cw.l21t = reinterpret_cast<float*>(lp);   // the FP32 update uses l21t, the 16-bit ones l16 + rs
+  cw.l16 = reinterpret_cast<_Float16*>(lp);
+  lp += sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX;
+  cw.rs = reinterpret_cast<float*>(lp);
+
+  // ---- 1. symmetric FP64 copy, norm, scaling, FP32 working matrix
+  hipLaunchKernelGGL(symmetrize_kernel, dim3(blocks(n, 32), blocks(n, 32)), dim3(32, 8), 0, s, layout, uplo, n,
+                     A, lda, S64);
+  hipLaunchKernelGGL(rowsum_abs_kernel, dim3((unsigned)n), dim3(256), 0, s, (int)kRowMajor, n,
+                     (const double*)S64, n, S.rowsum);
+  ZH_LAUNCH_CHECK();
+  S.anorm = reduce_host(S, n, S.rowsum, nullptr, 1);
+  hipLaunchKernelGGL(diag_scale_kernel, dim3(blocks(n)), dim3(256), 0, s, n, (const double*)S64, S.r);
+  ZH_LAUNCH_CHECK();
+
+  // ---- 2. factorization (shifted again while a pivot fails)
+  std::vector<int> ib(S.nblk);
+  double shift = 0.0;
+  for (int attempt = 0;; ++attempt) {
+    hipLaunchKernelGGL(equilibrate_kernel, dim3(blocks(n, 32), blocks(n, 32)), dim3(32, 8), 0, s, (int)kRowMajor,
+                       n, (const double*)S64, n, (const double*)S.r, (const double*)S.r, S.W);
+    ZH_LAUNCH_CHECK();
+    chol_factor(S, cw, opts.precision, shift);
+    ZH_CHECK(hipMemcpyAsync(ib.data(), cw.info_blk, sizeof(int) * S.nblk, hipMemcpyDeviceToHost, s));
+    ZH_CHECK(hipStreamSynchronize(s));
+    int first = 0;
+    for (int64_t b = 0; b < S.nblk && !first; ++b)
+      if (ib[b]) first = (int)(b * NB_MIX + ib[b]);
+    if (attempt == 0) res->info = first;
+    if (!first) break;
+    if (attempt >= 4) throw std::runtime_error("dposv_mixed: the shifted low-precision Cholesky still fails");
+    shift = shift == 0.0 ? 0x1p-10 : shift * 16.0;
+    res->fallback = std::max(res->fallback, 1);
+  }
+  ZH_LAUNCH_CHECK();
+
+  // ---- 3./4. groups of up to 16 right-hand sides
+  double* vec = static_cast<double*>(workspace(sizeof(double) * 4 * (size_t)n * MAXR, 12));
+  double* Bc = vec;
+  double* Xc = vec + (size_t)n * MAXR;
+  double* Rc = vec + 2 * (size_t)n * MAXR;
+  double* gw = nullptr;
+  double worst = 0.0;
+  std::vector<double> hm(2 * MAXR);
+  for (int64_t q0 = 0; q0 < nrhs; q0 += MAXR) {
+    const int nr = (int)std::min<int64_t>(MAXR, nrhs - q0);
+    // B, X column-major n x nr (ld n)
+    for (int q = 0; q < nr; ++q) {
+      const int64_t col = q0 + q;
+      const int64_t bstride = layout == kColMajor ? 1 : ldb;
+      const double* bcol = layout == kColMajor ? B + col * ldb : B + col;
+      hipLaunchKernelGGL(copy_f64_kernel, dim3(blocks(n)), dim3(256), 0, s, n, bcol, bstride, Bc + q * n);
+    }
+    ZH_LAUNCH_CHECK();
+    chol_solve_block(S, nr, Bc, n, Xc, n, 0);
+    std::vector<double> prev(nr, INFINITY), be(nr, INFINITY);
+    std::vector<int> state(nr, 0);   // 0 refining, 1 converged, 2 needs GMRES-IR
+    int it = 0;
+    for (;; ++it) {
+      // R = B - S X (FP64), per-column backward errors
+      ZH_CHECK(hipMemcpyAsync(Rc, Bc, sizeof(double) * n * nr, hipMemcpyDeviceToDevice, s));
+      gemm_f64_cm(kNoTrans, kNoTrans, n, nr, n, -1.0, S64, n, Xc, n, 1.0, Rc, n, s);
+      ZH_CHECK(hipMemsetAsync(cmax, 0, sizeof(double) * 2 * MAXR, s));
+      hipLaunchKernelGGL(colmax_kernel, dim3(64, nr), dim3(256), 0, s, n, (const double*)Rc, n, cmax);
+      hipLaunchKernelGGL(colmax_kernel, dim3(64, nr), dim3(256), 0, s, n, (const double*)Xc, n, cmax + MAXR);
+      ZH_LAUNCH_CHECK();
+      ZH_CHECK(hipMemcpyAsync(hm.data(), cmax, sizeof(double) * 2 * MAXR, hipMemcpyDeviceToHost, s));
+      ZH_CHECK(hipStreamSynchronize(s));
+      bool active = false;
+      for (int q = 0; q < nr; ++q) {
+        if (state[q] != 0) continue;
+        be[q] = hm[q] / (S.anorm * hm[MAXR + q] * (double)n * 0x1p-53);
+        if (be[q] <= opts.tol) state[q] = 1;
+        else if (!(be[q] == be[q]) || be[q] > 2.0 * prev[q] || (it >= 2 && be[q] > 0.5 * prev[q]) ||
+                 it >= opts.max_iter)
+          state[q] = 2;
+        else active = true;
+        prev[q] = be[q];
+      }
+      if (!active) break;
+      // X += M^-1 R on the columns still refining (the whole group is solved at once)
+      unsigned mask = 0;
+      for (int q = 0; q < nr; ++q)
+        if (state[q] == 0) mask |= 1u << q;
+      hipLaunchKernelGGL(pack_rows_kernel, dim3(blocks(n * nr)), dim3(256), 0, s, n, nr, (const double*)Rc, n,
+                         (const double*)S.r, S.ybuf);
+      ZH_LAUNCH_CHECK();
+      chol_trsm_block(S, nr, S.ybuf);
+      hipLaunchKernelGGL(unpack_add_masked_kernel, dim3(blocks(n * nr)), dim3(256), 0, s, n, nr,
+                         (const float*)S.ybuf, (const double*)S.r, Xc, n, mask);
+      ZH_LAUNCH_CHECK();
+    }
+    res->iterations = std::max(res->iterations, it);
+    for (int q = 0; q < nr; ++q) {
+      double* xq = Xc + q * n;
+      const double* bq = Bc + q * n;
+      if (state[q] == 2 && !opts.no_gmres) {
+        if (!gw) gw = static_cast<double*>(workspace(sizeof(double) * (size_t)(2 * opts.gmres_restart + 3) * n, 13));
+        double be2 = 0.0;
+        const int gi = gmres_ir(S, bq, xq, opts.gmres_restart, opts.gmres_max_iter, opts.tol, &be2, gw);
+        res->gmres_iterations = std::max(res->gmres_iterations, gi);
+        res->fallback = std::max(res->fallback, 1);
+        be[q] = be2;
+        if (be2 <= opts.tol) state[q] = 1;
+      }
+      if (state[q] != 1) {
+        // FP64 Cholesky of S (symmetric: the column-major view is S itself)
+        res->fallback = 2;
+        double* A64 = static_cast<double*>(workspace(sizeof(double) * (size_t)n * n, 14));
+        int* inf64 = cw.info_blk + S.nblk;
+        ZH_CHECK(hipMemcpyAsync(A64, S64, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+        ZH_CHECK(hipMemsetAsync(inf64, 0, sizeof(int), s));
+        potrf_f64_cm(kLower, n, A64, n, inf64, s);
+        ZH_CHECK(hipMemcpyAsync(xq, bq, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        trsm_f64_cm(kLeft, kLower, kNoTrans, kNonUnit, n, 1, 1.0, A64, n, xq, n, s);
+        trsm_f64_cm(kLeft, kLower, kTrans, kNonUnit, n, 1, 1.0, A64, n, xq, n, s);
+        double* rq = Rc + q * n;
+        residual(S, bq, xq, rq);
+        be[q] = backward_error(S, rq, xq);
+      }
+      worst = std::max(worst, be[q]);
+      const int64_t col = q0 + q;
+      const int64_t xstride = layout == kColMajor ? 1 : ldx;
+      double* xcol = layout == kColMajor ? X + col * ldx : X + col;
+      hipLaunchKernelGGL(store_f64_kernel, dim3(blocks(n)), dim3(256), 0, s, n, (const double*)xq, xcol, xstride);
+    }
+    ZH_LAUNCH_CHECK();
+  }
+  ZH_CHECK(hipStreamSynchronize(s));
+  res->backward_error = worst;
+  return 0;
 }
 
 }  // namespace zmlhip

@@ -9,13 +9,13 @@ namespace zmlhip {
 // C = beta C + alpha rs_i cs_j sum_k A_ik B_kj, FP32 accumulate (mp_gemm.hip).
 void mp_gemm_f16(int64_t m, int64_t n, int64_t K, float alpha, const _Float16* A, int64_t lda,
                  const _Float16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
-                 float* C, int64_t ldc, hipStream_t s);
+                 float* C, int64_t ldc, hipStream_t s, int lower = 0);
 void mp_gemm_bf16(int64_t m, int64_t n, int64_t K, float alpha, const __bf16* A, int64_t lda,
                   const __bf16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
-                  float* C, int64_t ldc, hipStream_t s);
+                  float* C, int64_t ldc, hipStream_t s, int lower = 0);
 void mp_gemm_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, int64_t lda,
                  const float* B, int64_t ldb, float beta, const float* rs, const float* cs, float* C,
-                 int64_t ldc, hipStream_t s);
+                 int64_t ldc, hipStream_t s, int lower = 0);
 
 // Precision of the trailing update of the mixed factorizations.
 enum LowPrec : int { kPrecF32 = 0, kPrecF16 = 1, kPrecBF16 = 2 };
@@ -46,6 +46,10 @@ struct LuOps {
   // driver's two alternating buffers when null)
   T* (*inv_store)(int64_t j0);
 };
+
+int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                const double* B, int64_t ldb, double* X, int64_t ldx, const zmlhip_ir_opts* o,
+                zmlhip_ir_result* res, hipStream_t s);
 
 int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t lda, const double* B,
                 int64_t ldb, double* X, int64_t ldx, const zmlhip_ir_opts* o, zmlhip_ir_result* res,

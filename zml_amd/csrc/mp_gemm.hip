@@ -9,6 +9,8 @@
 //     A is m x K row-major, B is given transposed (BT: n x K row-major).
 //   * FP32: v_mfma_f32_32x32x2_f32. A is m x K row-major, B is K x n
 //     row-major (its natural layout: one element per lane per MFMA).
+// With `lower`, tiles strictly above the diagonal of C are skipped (SYRK-style
+// updates of a lower triangle; the upper part of diagonal tiles is written).
 // A 256-thread workgroup computes 128 x 128 of C (2 x 2 waves of 64 x 64,
 // each 2 x 2 MFMA blocks of 32 x 32); K advances through double-buffered
 // LDS. Lane l of a 32 x 32 block holds column l % 32 and rows
@@ -100,7 +102,7 @@ template <typename H>
 __global__ void __launch_bounds__(NTH, 2)
 mp_gemm_h_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restrict__ A, int64_t lda,
                  const H* __restrict__ BT, int64_t ldb, float beta, const float* __restrict__ rs,
-                 const float* __restrict__ cs, float* __restrict__ C, int64_t ldc) {
+                 const float* __restrict__ cs, float* __restrict__ C, int64_t ldc, int lower) {
   typedef typename MfmaH<H>::vec V;
   __shared__ __attribute__((aligned(16))) H As[2][MT * LDH];
   __shared__ __attribute__((aligned(16))) H Bs[2][NTL * LDH];
@@ -108,6 +110,7 @@ mp_gemm_h_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restri
   int64_t tm, tn;
   tile_of(blockIdx.x, ntm, ntn, tm, tn);
   const int64_t row0 = tm * MT, col0 = tn * NTL;
+  if (lower && col0 > row0 + MT - 1) return;   // tile strictly above the diagonal
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
@@ -198,13 +201,14 @@ __global__ void __launch_bounds__(NTH, 2)
 mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __restrict__ A,
                  int64_t lda, const float* __restrict__ B, int64_t ldb, float beta,
                  const float* __restrict__ rs, const float* __restrict__ cs, float* __restrict__ C,
-                 int64_t ldc) {
+                 int64_t ldc, int lower) {
   __shared__ float As[2][MT * LDAF];
   __shared__ float Bs[2][BKF * LDBF];
   const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
   int64_t tm, tn;
   tile_of(blockIdx.x, ntm, ntn, tm, tn);
   const int64_t row0 = tm * MT, col0 = tn * NTL;
+  if (lower && col0 > row0 + MT - 1) return;   // tile strictly above the diagonal
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
@@ -276,31 +280,31 @@ mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __re
 
 void mp_gemm_f16(int64_t m, int64_t n, int64_t K, float alpha, const _Float16* A, int64_t lda,
                  const _Float16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
-                 float* C, int64_t ldc, hipStream_t s) {
+                 float* C, int64_t ldc, hipStream_t s, int lower) {
   if (m <= 0 || n <= 0) return;
   const int64_t tiles = ((m + MT - 1) / MT) * ((n + NTL - 1) / NTL);
   hipLaunchKernelGGL((mp_gemm_h_kernel<_Float16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
-                     A, lda, BT, ldb, beta, rs, cs, C, ldc);
+                     A, lda, BT, ldb, beta, rs, cs, C, ldc, lower);
   ZH_LAUNCH_CHECK();
 }
 
 void mp_gemm_bf16(int64_t m, int64_t n, int64_t K, float alpha, const __bf16* A, int64_t lda,
                   const __bf16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
-                  float* C, int64_t ldc, hipStream_t s) {
+                  float* C, int64_t ldc, hipStream_t s, int lower) {
   if (m <= 0 || n <= 0) return;
   const int64_t tiles = ((m + MT - 1) / MT) * ((n + NTL - 1) / NTL);
   hipLaunchKernelGGL((mp_gemm_h_kernel<__bf16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
-                     A, lda, BT, ldb, beta, rs, cs, C, ldc);
+                     A, lda, BT, ldb, beta, rs, cs, C, ldc, lower);
   ZH_LAUNCH_CHECK();
 }
 
 void mp_gemm_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, int64_t lda,
                  const float* B, int64_t ldb, float beta, const float* rs, const float* cs, float* C,
-                 int64_t ldc, hipStream_t s) {
+                 int64_t ldc, hipStream_t s, int lower) {
   if (m <= 0 || n <= 0) return;
   const int64_t tiles = ((m + MT - 1) / MT) * ((n + NTL - 1) / NTL);
   hipLaunchKernelGGL(mp_gemm_f_kernel, dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha, A, lda,
-                     B, ldb, beta, rs, cs, C, ldc);
+                     B, ldb, beta, rs, cs, C, ldc, lower);
   ZH_LAUNCH_CHECK();
 }
 

@@ -16,7 +16,7 @@ struct Buf {
   void* ptr = nullptr;
   size_t bytes = 0;
 };
-constexpr int kSlots = 16;
+constexpr int kSlots = 24;
 constexpr int kAux = 4, kEvents = 8;
 struct DeviceState {
   Buf slot[kSlots];
