@@ -129,6 +129,28 @@ int zmlhip_dposv_mixed(int layout, char uplo, int n, int nrhs, const double *a, 
                        const double *b, int ldb, double *x, int ldx, const zmlhip_ir_opts *opts,
                        zmlhip_ir_result *result, void *stream);
 
+/* Batched solves of many small independent systems (n <= 256 each), FP32
+ * factorization + FP64 refinement per system, all in one hipGraph launch.
+ * A plan records the systems (host arrays of sizes and of device pointers;
+ * every matrix column-major with ld = n, b and x column-major n x nrhs with
+ * ld = n; spd[i] = 1 selects elimination without interchanges for a
+ * symmetric positive definite A, whose full matrix must be stored), picks the
+ * thread-grid shape of each size bucket (by timing the candidates when
+ * `autotune` is set) and captures the graph; execute replays it on `stream`.
+ * opts: max_iter (default 10) and tol (default 1) are used. status (host,
+ * count ints) receives per system the refinement steps of its worst
+ * right-hand side, or -1 when the refinement did not reach the target. */
+typedef struct zmlhip_batch_plan zmlhip_batch_plan;
+int zmlhip_batch_plan_create(zmlhip_batch_plan **plan, int count, const int *n, const int *nrhs,
+                             const int *spd, const double *const *a, const double *const *b,
+                             double *const *x, const zmlhip_ir_opts *opts, int autotune, void *stream);
+int zmlhip_batch_plan_execute(zmlhip_batch_plan *plan, void *stream);
+int zmlhip_batch_plan_status(zmlhip_batch_plan *plan, int *status, void *stream);
+/* thread-grid side, values per thread side and system count of the 4 size
+ * buckets (n <= 32, 64, 128, 256) */
+int zmlhip_batch_plan_shapes(const zmlhip_batch_plan *plan, int *tg, int *r, int *count);
+int zmlhip_batch_plan_destroy(zmlhip_batch_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif

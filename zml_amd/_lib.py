@@ -64,6 +64,11 @@ SIGNATURES = {
                                     _p]),
     "zmlhip_dposv_mixed": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _c_int, _p, _c_int,
                                     _p, _p, _p]),
+    "zmlhip_batch_plan_create": (_c_int, [_p, _c_int, _p, _p, _p, _p, _p, _p, _p, _c_int, _p]),
+    "zmlhip_batch_plan_execute": (_c_int, [_p, _p]),
+    "zmlhip_batch_plan_status": (_c_int, [_p, _p, _p]),
+    "zmlhip_batch_plan_shapes": (_c_int, [_p, _p, _p, _p]),
+    "zmlhip_batch_plan_destroy": (_c_int, [_p]),
 }
 
 

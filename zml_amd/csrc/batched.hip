@@ -1,0 +1,435 @@
+// Batched mixed-precision solves of many small independent systems (n <= 256).
+//
+// No counterpart in zml (its getrf/potrf work on one matrix per call); per
+// system the arithmetic is the one of the large mixed solves: an FP32
+// factorization and FP64 iterative refinement against the FP64 A.
+//
+// A plan sorts the systems into size buckets (n <= 32, 64, 128, 256). Every
+// bucket runs one persistent kernel whose workgroups take systems from an
+// atomic work queue; a workgroup keeps its system's FP32 factor in registers
+// (a TG x TG thread grid, R x R values per thread, NMAX = TG R) from the
+// factorization through every refinement step, and reads the FP64 matrix
+// from HBM only for the residuals. The thread-grid shape of every bucket is
+// chosen by timing the candidate shapes on the bucket's own systems
+// (autotune), and the queue resets plus the bucket kernels are captured once
+// into a hipGraph, so that one graph launch solves the whole batch.
+//
+// General systems: LU with partial pivoting (largest |a|, lowest row on
+// ties) with virtual interchanges: a chosen pivot row is marked done and
+// stays where it is. SPD systems: the same elimination without interchanges
+// (the diagonal is the pivot), i.e. the LDL^T = Cholesky factor up to the
+// diagonal scaling, computed on the whole matrix.
+// Refinement per right-hand side: x = 0, r = b; repeat d = (LU)^-1 r in FP32,
+// x += d, r = b - A x in FP64, until ||r||_inf <= tol ||A||_inf ||x||_inf n u;
+// a system whose backward error stops halving, or that runs out of steps,
+// reports -1.
+#include "zmlhip_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "../../include/zmlhip.h"
+
+namespace zmlhip {
+namespace {
+
+struct SysDesc {
+  const double* a;   // column-major, ld n
+  const double* b;   // column-major n x nrhs, ld n
+  double* x;
+  int n, nrhs, spd, pad;
+};
+
+struct BatchArgs {
+  const SysDesc* sys;
+  const int* list;   // systems of this bucket
+  int count;
+  int* counter;      // work-queue head (reset before every run)
+  int* status;       // per system: refinement steps of its worst right-hand side, or -1
+  int max_iter;
+  double tol;
+};
+
+__device__ __forceinline__ uint64_t bkey(float v, int r) {
+  if (v != v) return (uint64_t)(0x7fffffffu - (uint32_t)r);
+  return ((uint64_t)__float_as_uint(v) << 32) | (0xffffffffu - (uint32_t)r);
+}
+__device__ __forceinline__ int bkey_row(uint64_t k) {
+  const uint32_t lo = (uint32_t)k;
+  return lo >= 0x80000000u ? (int)(0xffffffffu - lo) : (int)(0x7fffffffu - lo);
+}
+
+template <int NW>
+__device__ __forceinline__ double block_max(double v, double* red, int t) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double m = red[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) m = fmax(m, red[w]);
+  __syncthreads();
+  return m;
+}
+
+template <int TG, int R>
+__global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
+  constexpr int NT = TG * TG, NMAX = TG * R, NW = (NT + 63) / 64;
+  __shared__ uint64_t wbest[NW];
+  __shared__ float lcol[NMAX], urow[NMAX], bw[NMAX], ud[NMAX], xf[NMAX];
+  __shared__ int prow[NMAX], pos[NMAX];
+  __shared__ double xv[NMAX], rv[NMAX], red[NW];
+  __shared__ int s_idx, s_bad;
+  const int t = threadIdx.x, tx = t % TG, ty = t / TG, lane = t & 63, wv = t >> 6;
+  for (;;) {
+    if (t == 0) s_idx = atomicAdd(g.counter, 1);
+    __syncthreads();
+    const int idx = s_idx;
+    __syncthreads();
+    if (idx >= g.count) break;
+    const int sid = g.list[idx];
+    const SysDesc d = g.sys[sid];
+    const int n = d.n;
+
+    // ---- FP32 factorization, in registers
+    float a[R][R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int row = ty + TG * i, col = tx + TG * j;
+        a[i][j] = (row < n && col < n) ? (float)d.a[row + (int64_t)col * n] : 0.0f;
+      }
+    unsigned done = 0;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      if (ty + TG * i >= n) done |= 1u << i;
+    if (t == 0) s_bad = 0;
+#pragma unroll
+    for (int jj = 0; jj < R; ++jj) {
+      for (int kq = 0; kq < TG; ++kq) {
+        const int k = jj * TG + kq;
+        if (k >= n) break;
+        int p = k;
+        if (!d.spd) {
+          uint64_t best = 0;
+          if (tx == kq) {
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+              if (!((done >> i) & 1u)) {
+                const uint64_t key = bkey(fabsf(a[i][jj]), ty + TG * i);
+                best = key > best ? key : best;
+              }
+          }
+#pragma unroll
+          for (int off = TG; off < 64; off <<= 1) {
+            const uint64_t o = __shfl_xor(best, off);
+            best = o > best ? o : best;
+          }
+          if (lane == kq) wbest[wv] = best;
+        }
+        // also orders this step's urow writes after the previous step's reads
+        __syncthreads();
+        if (!d.spd) {
+          uint64_t gk = wbest[0];
+#pragma unroll
+          for (int w = 1; w < NW; ++w) gk = wbest[w] > gk ? wbest[w] : gk;
+          p = bkey_row(gk);
+        }
+        const int pt = p % TG, pi = p / TG;
+        if (ty == pt) {
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+            if (i == pi) {
+#pragma unroll
+              for (int j = 0; j < R; ++j) urow[tx + TG * j] = a[i][j];
+            }
+          done |= 1u << pi;
+        }
+        if (t == 0) prow[k] = p;
+        __syncthreads();
+        const float pv = urow[k];
+        const bool ok = pv != 0.0f && pv == pv;
+        if (t == 0) {
+          ud[k] = pv;
+          if (!ok) s_bad = 1;
+        }
+        const bool recip = fabsf(pv) >= 0x1p-126f;
+        const float rp = 1.0f / pv;
+        if (tx == kq) {
+#pragma unroll
+          for (int i = 0; i < R; ++i) {
+            float l = 0.0f;
+            if (!((done >> i) & 1u) && ok) {
+              l = recip ? a[i][jj] * rp : a[i][jj] / pv;
+              a[i][jj] = l;
+            }
+            lcol[ty + TG * i] = l;
+          }
+        }
+        __syncthreads();
+        float li[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) li[i] = lcol[ty + TG * i];
+#pragma unroll
+        for (int j = jj; j < R; ++j) {
+          const float u = urow[tx + TG * j];
+          if (j > jj || tx > kq) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) a[i][j] -= li[i] * u;
+          }
+        }
+      }
+    }
+    if (t < n) pos[prow[t]] = t;
+    __syncthreads();
+
+    // ---- FP64 refinement of every right-hand side
+    int worst = s_bad ? -1 : 0;
+    double anorm = 0.0;
+    for (int q = 0; q < d.nrhs && worst >= 0; ++q) {
+      const double* b = d.b + (int64_t)q * n;
+      if (t < NMAX) {
+        xv[t] = 0.0;
+        rv[t] = t < n ? b[t] : 0.0;
+      }
+      __syncthreads();
+      double prev = INFINITY;
+      int it = 0;
+      for (;;) {
+        // d = U^-1 L^-1 P r in FP32; rows keep their physical place
+        if (t < NMAX) bw[t] = (float)rv[t];
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < R; ++jj) {
+          for (int kq = 0; kq < TG; ++kq) {
+            const int k = jj * TG + kq;
+            if (k >= n) break;
+            const float yk = bw[prow[k]];
+            if (tx == kq) {
+#pragma unroll
+              for (int i = 0; i < R; ++i) {
+                const int r = ty + TG * i;
+                if (r < n && pos[r] > k) bw[r] -= a[i][jj] * yk;
+              }
+            }
+            __syncthreads();
+          }
+        }
+#pragma unroll
+        for (int jj = R - 1; jj >= 0; --jj) {
+          for (int kq = TG - 1; kq >= 0; --kq) {
+            const int k = jj * TG + kq;
+            if (k >= n) continue;
+            const float xk = bw[prow[k]] / ud[k];
+            if (t == 0) xf[k] = xk;
+            if (tx == kq) {
+#pragma unroll
+              for (int i = 0; i < R; ++i) {
+                const int r = ty + TG * i;
+                if (r < n && pos[r] < k) bw[r] -= a[i][jj] * xk;
+              }
+            }
+            __syncthreads();
+          }
+        }
+        if (t < n) xv[t] += (double)xf[t];
+        __syncthreads();
+        // r = b - A x, ||A||_inf on the first pass
+        double ri = 0.0, ai = 0.0;
+        if (t < n) {
+          double acc = b[t], asum = 0.0;
+          for (int j = 0; j < n; ++j) {
+            const double v = d.a[t + (int64_t)j * n];
+            acc -= v * xv[j];
+            asum += fabs(v);
+          }
+          rv[t] = acc;
+          ri = fabs(acc);
+          ai = asum;
+        }
+        if (anorm == 0.0) anorm = block_max<NW>(ai, red, t);
+        const double rmax = block_max<NW>(ri, red, t);
+        const double xmax = block_max<NW>(t < n ? fabs(xv[t]) : 0.0, red, t);
+        ++it;
+        const double be = rmax / (anorm * xmax * (double)n * 0x1p-53);
+        if (be <= g.tol) break;
+        if (!(be == be) || it >= g.max_iter || (it >= 2 && be > 0.5 * prev)) {
+          it = -1;
+          break;
+        }
+        prev = be;
+      }
+      if (t < n) d.x[(int64_t)q * n + t] = xv[t];
+      worst = it < 0 ? -1 : std::max(worst, it);
+      __syncthreads();
+    }
+    if (t == 0) g.status[sid] = worst;
+    __syncthreads();
+  }
+}
+
+__global__ void reset_counters_kernel(int* c, int k) {
+  if ((int)threadIdx.x < k) c[threadIdx.x] = 0;
+}
+
+// ---- variants and buckets -------------------------------------------------------------
+struct Variant {
+  int tg, r;
+  void (*kernel)(BatchArgs);
+};
+const Variant kVariants[] = {
+    {8, 4, batched_solve_kernel<8, 4>},    // n <= 32, one wave
+    {8, 8, batched_solve_kernel<8, 8>},    // n <= 64, one wave
+    {16, 4, batched_solve_kernel<16, 4>},  // n <= 64, four waves
+    {16, 8, batched_solve_kernel<16, 8>},  // n <= 128
+    {32, 4, batched_solve_kernel<32, 4>},  // n <= 128, sixteen waves
+    {32, 8, batched_solve_kernel<32, 8>},  // n <= 256
+};
+constexpr int kBuckets = 4;
+const int kBucketMax[kBuckets] = {32, 64, 128, 256};
+const std::vector<int> kCandidates[kBuckets] = {{0, 1, 2}, {1, 2}, {3, 4}, {5}};
+
+}  // namespace
+
+struct BatchPlan {
+  int count = 0;
+  SysDesc* d_sys = nullptr;
+  int* d_lists = nullptr;
+  int* d_counters = nullptr;
+  int* d_status = nullptr;
+  int bucket_count[kBuckets] = {};
+  int bucket_offset[kBuckets] = {};
+  int variant[kBuckets] = {};
+  unsigned grid[kBuckets] = {};
+  BatchArgs args[kBuckets];
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~BatchPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void* p : {(void*)d_sys, (void*)d_lists, (void*)d_counters, (void*)d_status})
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+unsigned resident_grid(int v) {
+  int per_cu = 0;
+  const int nt = kVariants[v].tg * kVariants[v].tg;
+  ZH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kVariants[v].kernel),
+                                                        nt, 0));
+  return (unsigned)(std::max(per_cu, 1) * device_cu_count());
+}
+
+void launch_bucket(const BatchPlan& P, int b, int v, hipStream_t s) {
+  const int nt = kVariants[v].tg * kVariants[v].tg;
+  const unsigned grid = std::min<unsigned>(P.grid[b], (unsigned)std::max(1, P.bucket_count[b]));
+  BatchArgs args = P.args[b];
+  void* kargs[] = {&args};
+  ZH_CHECK(hipLaunchKernel(reinterpret_cast<const void*>(kVariants[v].kernel), dim3(grid), dim3(nt), kargs, 0, s));
+}
+
+}  // namespace
+
+BatchPlan* batch_plan_create(int count, const int* n, const int* nrhs, const int* spd,
+                             const double* const* a, const double* const* b, double* const* x,
+                             const zmlhip_ir_opts* opts, int autotune, hipStream_t s) {
+  std::unique_ptr<BatchPlan> P(new BatchPlan);
+  P->count = count;
+  std::vector<SysDesc> h(count);
+  std::vector<std::vector<int>> lists(kBuckets);
+  for (int i = 0; i < count; ++i) {
+    h[i] = SysDesc{a[i], b[i], x[i], n[i], nrhs[i], spd ? (spd[i] ? 1 : 0) : 0, 0};
+    int bk = 0;
+    while (bk < kBuckets - 1 && n[i] > kBucketMax[bk]) ++bk;
+    lists[bk].push_back(i);
+  }
+  std::vector<int> flat;
+  for (int bk = 0; bk < kBuckets; ++bk) {
+    // larger systems first inside a bucket: the queue ends on the short ones
+    std::stable_sort(lists[bk].begin(), lists[bk].end(), [&](int u, int v) { return n[u] > n[v]; });
+    P->bucket_offset[bk] = (int)flat.size();
+    P->bucket_count[bk] = (int)lists[bk].size();
+    flat.insert(flat.end(), lists[bk].begin(), lists[bk].end());
+  }
+  ZH_CHECK(hipMalloc(&P->d_sys, sizeof(SysDesc) * std::max(count, 1)));
+  ZH_CHECK(hipMalloc(&P->d_lists, sizeof(int) * std::max<size_t>(flat.size(), 1)));
+  ZH_CHECK(hipMalloc(&P->d_counters, sizeof(int) * 64));
+  ZH_CHECK(hipMalloc(&P->d_status, sizeof(int) * std::max(count, 1)));
+  ZH_CHECK(hipMemcpyAsync(P->d_sys, h.data(), sizeof(SysDesc) * count, hipMemcpyHostToDevice, s));
+  if (!flat.empty())
+    ZH_CHECK(hipMemcpyAsync(P->d_lists, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice, s));
+  const int max_iter = opts && opts->max_iter > 0 ? opts->max_iter : 10;
+  const double tol = opts && opts->tol > 0.0 ? opts->tol : 1.0;
+  for (int bk = 0; bk < kBuckets; ++bk) {
+    P->args[bk] = BatchArgs{P->d_sys, P->d_lists + P->bucket_offset[bk], P->bucket_count[bk],
+                            P->d_counters + bk, P->d_status, max_iter, tol};
+    P->variant[bk] = kCandidates[bk].back();
+  }
+  // autotune: time every candidate shape on the bucket's own systems
+  for (int bk = 0; bk < kBuckets; ++bk) {
+    const auto& cand = kCandidates[bk];
+    if (P->bucket_count[bk] == 0) continue;
+    float best = INFINITY;
+    if (!autotune || cand.size() == 1) {
+      P->grid[bk] = resident_grid(P->variant[bk]);
+      continue;
+    }
+    hipEvent_t e0, e1;
+    ZH_CHECK(hipEventCreate(&e0));
+    ZH_CHECK(hipEventCreate(&e1));
+    for (int v : cand) {
+      P->grid[bk] = resident_grid(v);
+      float ms = 0.0f;
+      for (int rep = 0; rep < 2; ++rep) {   // the first run warms up
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, P->d_counters, kBuckets);
+        ZH_CHECK(hipEventRecord(e0, s));
+        launch_bucket(*P, bk, v, s);
+        ZH_CHECK(hipEventRecord(e1, s));
+        ZH_CHECK(hipEventSynchronize(e1));
+        ZH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      }
+      if (ms < best) {
+        best = ms;
+        P->variant[bk] = v;
+      }
+    }
+    P->grid[bk] = resident_grid(P->variant[bk]);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  // one graph: reset the queues, then every bucket
+  hipStream_t cs;
+  ZH_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  ZH_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, cs, P->d_counters, kBuckets);
+  for (int bk = kBuckets - 1; bk >= 0; --bk)
+    if (P->bucket_count[bk] > 0) launch_bucket(*P, bk, P->variant[bk], cs);
+  ZH_CHECK(hipStreamEndCapture(cs, &P->graph));
+  ZH_CHECK(hipStreamDestroy(cs));
+  ZH_CHECK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+  ZH_CHECK(hipStreamSynchronize(s));
+  return P.release();
+}
+
+void batch_plan_execute(BatchPlan* P, hipStream_t s) { ZH_CHECK(hipGraphLaunch(P->exec, s)); }
+
+void batch_plan_status(BatchPlan* P, int* status, hipStream_t s) {
+  ZH_CHECK(hipMemcpyAsync(status, P->d_status, sizeof(int) * P->count, hipMemcpyDeviceToHost, s));
+  ZH_CHECK(hipStreamSynchronize(s));
+}
+
+void batch_plan_variants(const BatchPlan* P, int* tg, int* r, int* counts) {
+  for (int bk = 0; bk < kBuckets; ++bk) {
+    tg[bk] = kVariants[P->variant[bk]].tg;
+    r[bk] = kVariants[P->variant[bk]].r;
+    counts[bk] = P->bucket_count[bk];
+  }
+}
+
+void batch_plan_destroy(BatchPlan* P) { delete P; }
+
+}  // namespace zmlhip

@@ -423,6 +423,57 @@ int zmlhip_dposv_mixed(int layout, char uplo, int n, int nrhs, const double* a, 
   });
 }
 
+int zmlhip_batch_plan_create(zmlhip_batch_plan** plan, int count, const int* n, const int* nrhs,
+                             const int* spd, const double* const* a, const double* const* b, double* const* x,
+                             const zmlhip_ir_opts* opts, int autotune, void* stream) {
+  return guarded([&] {
+    if (!plan) return -1;
+    *plan = nullptr;
+    if (count < 0) return -2;
+    if (count > 0 && (!n || !nrhs || !a || !b || !x)) return -3;
+    for (int i = 0; i < count; ++i) {
+      if (n[i] < 1 || n[i] > 256) return -3;
+      if (nrhs[i] < 1) return -4;
+      if (!a[i] || !b[i] || !x[i]) return -6;
+    }
+    *plan = reinterpret_cast<zmlhip_batch_plan*>(
+        batch_plan_create(count, n, nrhs, spd, a, b, x, opts, autotune, as_stream(stream)));
+    return 0;
+  });
+}
+
+int zmlhip_batch_plan_execute(zmlhip_batch_plan* plan, void* stream) {
+  return guarded([&] {
+    if (!plan) return -1;
+    batch_plan_execute(reinterpret_cast<BatchPlan*>(plan), as_stream(stream));
+    return 0;
+  });
+}
+
+int zmlhip_batch_plan_status(zmlhip_batch_plan* plan, int* status, void* stream) {
+  return guarded([&] {
+    if (!plan) return -1;
+    if (!status) return -2;
+    batch_plan_status(reinterpret_cast<BatchPlan*>(plan), status, as_stream(stream));
+    return 0;
+  });
+}
+
+int zmlhip_batch_plan_shapes(const zmlhip_batch_plan* plan, int* tg, int* r, int* count) {
+  return guarded([&] {
+    if (!plan || !tg || !r || !count) return -1;
+    batch_plan_variants(reinterpret_cast<const BatchPlan*>(plan), tg, r, count);
+    return 0;
+  });
+}
+
+int zmlhip_batch_plan_destroy(zmlhip_batch_plan* plan) {
+  return guarded([&] {
+    if (plan) batch_plan_destroy(reinterpret_cast<BatchPlan*>(plan));
+    return 0;
+  });
+}
+
 // =============================================================================
 // Host-pointer CBLAS (include/cblas.h): synchronous, errors ignored as zml does
 // =============================================================================

@@ -18,6 +18,8 @@
 #include <stdexcept>
 #include <string>
 
+struct zmlhip_ir_opts;
+
 namespace zmlhip {
 
 enum Layout : int { kRowMajor = 101, kColMajor = 102 };
@@ -150,5 +152,15 @@ void transpose_square_inplace(int64_t n, double* a, int64_t ld, hipStream_t s);
 // Out-of-place: dst (cols x rows, ldd) = src^T (src rows x cols, lds; row-major view).
 void transpose_copy(int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst,
                     int64_t ldd, hipStream_t s);
+
+// Batched small solves (batched.hip)
+struct BatchPlan;
+BatchPlan* batch_plan_create(int count, const int* n, const int* nrhs, const int* spd,
+                             const double* const* a, const double* const* b, double* const* x,
+                             const zmlhip_ir_opts* opts, int autotune, hipStream_t s);
+void batch_plan_execute(BatchPlan* P, hipStream_t s);
+void batch_plan_status(BatchPlan* P, int* status, hipStream_t s);
+void batch_plan_variants(const BatchPlan* P, int* tg, int* r, int* counts);
+void batch_plan_destroy(BatchPlan* P);
 
 }  // namespace zmlhip
