@@ -416,7 +416,8 @@ def bench_batched(args):
             worst = max(worst, np.abs(Bi[:, q] - Ai @ Xi[:, q]).max() / (an * np.abs(Xi[:, q]).max() * n * 2.0 ** -53))
     flops = float(np.sum(np.where(spd, ns ** 3 / 3.0, 2.0 * ns ** 3 / 3.0)))
     steps = np.maximum(status, 0)
-    bytes_ = float(np.sum(8.0 * ns * ns * (1 + steps) + 16.0 * ns * nr))
+    # A read for the inversion and once per residual (up to `steps` per right-hand side)
+    bytes_ = float(np.sum(8.0 * ns * ns * (1 + nr * steps) + 16.0 * ns * nr))
     zh.zmlhip_batch_plan_destroy(plan)
     out = {
         "metric": "batched uneven solves, systems/s (TFLOP/s) at FP64 backward error",

@@ -14,14 +14,16 @@
 // (autotune), and the queue resets plus the bucket kernels are captured once
 // into a hipGraph, so that one graph launch solves the whole batch.
 //
-// General systems: LU with partial pivoting (largest |a|, lowest row on
-// ties) with virtual interchanges: a chosen pivot row is marked done and
-// stays where it is. SPD systems: the same elimination without interchanges
-// (the diagonal is the pivot), i.e. the LDL^T = Cholesky factor up to the
-// diagonal scaling, computed on the whole matrix.
-// Refinement per right-hand side: x = 0, r = b; repeat d = (LU)^-1 r in FP32,
-// x += d, r = b - A x in FP64, until ||r||_inf <= tol ||A||_inf ||x||_inf n u;
-// a system whose backward error stops halving, or that runs out of steps,
+// The FP32 factorization is a Gauss-Jordan inversion in place: general
+// systems pivot on the largest |a| of the column among the rows not yet
+// chosen (lowest row on ties), with virtual interchanges (a pivot row is
+// marked done and stays where it is); SPD systems pivot on the diagonal. The
+// result M holds inv(A) up to that row order, inv(A)[k][l] =
+// M[prow[k]][invp[l]], so every later solve is one parallel matrix-vector
+// product instead of two sequences of n dependent steps.
+// Refinement per right-hand side: x = 0, r = b; repeat x += inv32(A) r,
+// r = b - A x in FP64, until ||r||_inf <= tol ||A||_inf ||x||_inf n u; a
+// system whose backward error stops halving, or that runs out of steps,
 // reports -1.
 #include "zmlhip_internal.hpp"
 
@@ -77,8 +79,8 @@ template <int TG, int R>
 __global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
   constexpr int NT = TG * TG, NMAX = TG * R, NW = (NT + 63) / 64;
   __shared__ uint64_t wbest[NW];
-  __shared__ float lcol[NMAX], urow[NMAX], bw[NMAX], ud[NMAX], xf[NMAX];
-  __shared__ int prow[NMAX], pos[NMAX];
+  __shared__ float lcol[NMAX], urow[NMAX], cv[NMAX], yv[NMAX];
+  __shared__ int prow[NMAX];
   __shared__ double xv[NMAX], rv[NMAX], red[NW];
   __shared__ int s_idx, s_bad;
   const int t = threadIdx.x, tx = t % TG, ty = t / TG, lane = t & 63, wv = t >> 6;
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
     const SysDesc d = g.sys[sid];
     const int n = d.n;
 
-    // ---- FP32 factorization, in registers
+    // ---- FP32 Gauss-Jordan inversion, in registers
     float a[R][R];
 #pragma unroll
     for (int i = 0; i < R; ++i)
@@ -129,7 +131,7 @@ __global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
           }
           if (lane == kq) wbest[wv] = best;
         }
-        // also orders this step's urow writes after the previous step's reads
+        // also orders this step's urow / lcol writes after the previous step's reads
         __syncthreads();
         if (!d.spd) {
           uint64_t gk = wbest[0];
@@ -138,7 +140,7 @@ __global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
           p = bkey_row(gk);
         }
         const int pt = p % TG, pi = p / TG;
-        if (ty == pt) {
+        if (ty == pt) {   // the pivot row
 #pragma unroll
           for (int i = 0; i < R; ++i)
             if (i == pi) {
@@ -147,43 +149,41 @@ __global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
             }
           done |= 1u << pi;
         }
+        if (tx == kq) {   // column k: the multipliers of every other row
+#pragma unroll
+          for (int i = 0; i < R; ++i) lcol[ty + TG * i] = (ty + TG * i == p) ? 0.0f : a[i][jj];
+        }
         if (t == 0) prow[k] = p;
         __syncthreads();
         const float pv = urow[k];
-        const bool ok = pv != 0.0f && pv == pv;
-        if (t == 0) {
-          ud[k] = pv;
-          if (!ok) s_bad = 1;
-        }
-        const bool recip = fabsf(pv) >= 0x1p-126f;
+        if (t == 0 && !(pv != 0.0f && pv == pv)) s_bad = 1;
         const float rp = 1.0f / pv;
+        float fi[R], uj[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) fi[i] = lcol[ty + TG * i];
+#pragma unroll
+        for (int j = 0; j < R; ++j) uj[j] = urow[tx + TG * j] * rp;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+          for (int j = 0; j < R; ++j) a[i][j] -= fi[i] * uj[j];
+        if (ty == pt) {
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+            if (i == pi) {
+#pragma unroll
+              for (int j = 0; j < R; ++j) a[i][j] = uj[j];
+            }
+        }
         if (tx == kq) {
 #pragma unroll
-          for (int i = 0; i < R; ++i) {
-            float l = 0.0f;
-            if (!((done >> i) & 1u) && ok) {
-              l = recip ? a[i][jj] * rp : a[i][jj] / pv;
-              a[i][jj] = l;
-            }
-            lcol[ty + TG * i] = l;
-          }
-        }
-        __syncthreads();
-        float li[R];
-#pragma unroll
-        for (int i = 0; i < R; ++i) li[i] = lcol[ty + TG * i];
-#pragma unroll
-        for (int j = jj; j < R; ++j) {
-          const float u = urow[tx + TG * j];
-          if (j > jj || tx > kq) {
-#pragma unroll
-            for (int i = 0; i < R; ++i) a[i][j] -= li[i] * u;
-          }
+          for (int i = 0; i < R; ++i) a[i][jj] = (ty + TG * i == p) ? rp : -fi[i] * rp;
         }
       }
     }
-    if (t < n) pos[prow[t]] = t;
     __syncthreads();
+    // a now holds M with inv(A)[k][l] = M[prow[k]][invp[l]]: x = inv(A) r is
+    // c_j = r[prow[j]], y = M c, x_k = y[prow[k]]
 
     // ---- FP64 refinement of every right-hand side
     int worst = s_bad ? -1 : 0;
@@ -198,55 +198,38 @@ __global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
       double prev = INFINITY;
       int it = 0;
       for (;;) {
-        // d = U^-1 L^-1 P r in FP32; rows keep their physical place
-        if (t < NMAX) bw[t] = (float)rv[t];
+        if (t < NMAX) cv[t] = t < n ? (float)rv[prow[t]] : 0.0f;
         __syncthreads();
+        float acc[R];
 #pragma unroll
-        for (int jj = 0; jj < R; ++jj) {
-          for (int kq = 0; kq < TG; ++kq) {
-            const int k = jj * TG + kq;
-            if (k >= n) break;
-            const float yk = bw[prow[k]];
-            if (tx == kq) {
+        for (int i = 0; i < R; ++i) {
+          float s = 0.0f;
 #pragma unroll
-              for (int i = 0; i < R; ++i) {
-                const int r = ty + TG * i;
-                if (r < n && pos[r] > k) bw[r] -= a[i][jj] * yk;
-              }
-            }
-            __syncthreads();
-          }
+          for (int j = 0; j < R; ++j) s = fmaf(a[i][j], cv[tx + TG * j], s);
+          acc[i] = s;
         }
 #pragma unroll
-        for (int jj = R - 1; jj >= 0; --jj) {
-          for (int kq = TG - 1; kq >= 0; --kq) {
-            const int k = jj * TG + kq;
-            if (k >= n) continue;
-            const float xk = bw[prow[k]] / ud[k];
-            if (t == 0) xf[k] = xk;
-            if (tx == kq) {
+        for (int off = TG / 2; off > 0; off >>= 1)
 #pragma unroll
-              for (int i = 0; i < R; ++i) {
-                const int r = ty + TG * i;
-                if (r < n && pos[r] < k) bw[r] -= a[i][jj] * xk;
-              }
-            }
-            __syncthreads();
-          }
+          for (int i = 0; i < R; ++i) acc[i] += __shfl_xor(acc[i], off);
+        if (tx == 0) {
+#pragma unroll
+          for (int i = 0; i < R; ++i) yv[ty + TG * i] = acc[i];
         }
-        if (t < n) xv[t] += (double)xf[t];
+        __syncthreads();
+        if (t < n) xv[t] += (double)yv[prow[t]];
         __syncthreads();
         // r = b - A x, ||A||_inf on the first pass
         double ri = 0.0, ai = 0.0;
         if (t < n) {
-          double acc = b[t], asum = 0.0;
+          double accd = b[t], asum = 0.0;
           for (int j = 0; j < n; ++j) {
             const double v = d.a[t + (int64_t)j * n];
-            acc -= v * xv[j];
+            accd -= v * xv[j];
             asum += fabs(v);
           }
-          rv[t] = acc;
-          ri = fabs(acc);
+          rv[t] = accd;
+          ri = fabs(accd);
           ai = asum;
         }
         if (anorm == 0.0) anorm = block_max<NW>(ai, red, t);
