@@ -5,7 +5,7 @@
 // transposed in place (square) or through a workspace copy around it.
 //
 // Structure (right-looking, blocked):
-//   for each block column of NB = 256:
+//   for each block column of NB = 512:
 //     factor the (m - j) x 256 panel in leaves of 64 columns:
 //       leaf: one persistent kernel over all panel rows (below)
 //       leaf swaps -> other panel columns, TRSM + GEMM inside the panel
@@ -29,7 +29,7 @@
 
 namespace zmlhip {
 
-constexpr int NB_F64 = 256;     // block column width of the FP64 driver
+constexpr int NB_F64 = 512;     // block column width of the FP64 driver
 
 // Row-major wrappers over the column-major kernels (zml's own mappings:
 // gemm.zig:48-65, trsm.zig:48-62).

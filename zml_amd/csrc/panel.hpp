@@ -121,12 +121,18 @@ __device__ __forceinline__ T pick(const T (&a)[W], int c) {
   return out;
 }
 
+// The empty asm statements keep the compiler from hoisting every group's
+// pivot-row reads (LDS) ahead of the first group: only 8 values of u are live
+// at a time, which keeps the FP64 leaf within two waves per SIMD.
+#define ZH_GROUP_FENCE() asm volatile("" ::: "memory")
 template <typename T>
 __device__ __forceinline__ void update_row(T (&a)[W], T l, const T* __restrict__ u, int k) {
-  update_group<0>(a, l, u, k); update_group<1>(a, l, u, k); update_group<2>(a, l, u, k);
-  update_group<3>(a, l, u, k); update_group<4>(a, l, u, k); update_group<5>(a, l, u, k);
-  update_group<6>(a, l, u, k); update_group<7>(a, l, u, k);
+  update_group<0>(a, l, u, k); ZH_GROUP_FENCE(); update_group<1>(a, l, u, k); ZH_GROUP_FENCE();
+  update_group<2>(a, l, u, k); ZH_GROUP_FENCE(); update_group<3>(a, l, u, k); ZH_GROUP_FENCE();
+  update_group<4>(a, l, u, k); ZH_GROUP_FENCE(); update_group<5>(a, l, u, k); ZH_GROUP_FENCE();
+  update_group<6>(a, l, u, k); ZH_GROUP_FENCE(); update_group<7>(a, l, u, k);
 }
+#undef ZH_GROUP_FENCE
 
 // Leaf LU: rows [0, M) x columns [0, Wn) of the row-major panel P (Wn <= 64).
 // ipiv[k] = piv_base + pivot row + 1 (1-based, global); *info gets
@@ -151,6 +157,8 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
   __shared__ int64_t s_p;
   __shared__ int s_nan, s_abort;
   __shared__ int32_t s_piv[W];
+  __shared__ T stage[MULTI ? NW : 1][W], stage_k[W];
+  __shared__ int s_knan;
 
   T a[W];
 #pragma unroll
@@ -178,6 +186,19 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
       s_v[wv] = wmax;
       s_i[wv] = hit ? (int64_t)g * PT + wv * 64 + (__ffsll((unsigned long long)hit) - 1) : INT64_MAX;
     }
+    if (MULTI) {
+      // stage the wave's candidate row and row k for publishing (the barrier
+      // below orders these writes before the publishing wave reads them)
+      if (hit && lane == __ffsll((unsigned long long)hit) - 1) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) stage[wv][c] = a[c];
+      }
+      if (own && i == k) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) stage_k[c] = a[c];
+        s_knan = (ak != ak);
+      }
+    }
     __syncthreads();
     T bv = s_v[0];
     int64_t bi = s_i[0];
@@ -204,16 +225,16 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
       const uint32_t tag = tag_base + (uint32_t)k + 1u;
       uint64_t* my_rec = ws.rec + ((size_t)slot * GMAX + g) * REC;
       uint64_t* rkrec = ws.rowk + (size_t)slot * REC;
-      // publish: candidate row and row k straight from their owners' registers
-      if (own && i == bi) {
-        st64(my_rec, tagged(tag, (uint32_t)i));
-#pragma unroll
-        for (int c = 0; c < W; ++c) st_val<T>(my_rec + 2 + 2 * c, tag, a[c]);
+      // publish the candidate row and row k from the staging rows, one value
+      // per lane (lane c stores column c)
+      const int64_t base_row = (int64_t)g * PT;
+      if (bi != INT64_MAX && wv == (int)((bi - base_row) >> 6)) {
+        st_val<T>(my_rec + 2 + 2 * lane, tag, stage[wv][lane]);
+        if (lane == 0) st64(my_rec, tagged(tag, (uint32_t)bi));
       }
-      if (own && i == k) {
-        st64(rkrec, tagged(tag, (ak != ak) ? 1u : 0u));
-#pragma unroll
-        for (int c = 0; c < W; ++c) st_val<T>(rkrec + 2 + 2 * c, tag, a[c]);
+      if (k >= base_row && k < base_row + PT && wv == (int)((k - base_row) >> 6)) {
+        st_val<T>(rkrec + 2 + 2 * lane, tag, stage_k[lane]);
+        if (lane == 0) st64(rkrec, tagged(tag, s_knan ? 1u : 0u));
       }
       if (t == 0) st_val<T>(ws.val + ((size_t)slot * GMAX + g) * 2, tag, bi == INT64_MAX ? T(-1) : bv);
       if (wv == 0) {
@@ -307,7 +328,7 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
 }
 
 // ---- row interchanges through the composed permutation ----------------------
-constexpr int PERM_MAX = 512;   // pairs (2 x block of 256 interchanges)
+constexpr int PERM_MAX = 1024;  // pairs (2 x block of 512 interchanges)
 
 // Composes the interchanges i <-> ipiv[i]-1 for i = k1..k2 (ascending, every
 // ipiv[i]-1 >= i) into a list of moves dst <- src over the rows touched.
@@ -365,7 +386,7 @@ __global__ void compose_perm_kernel(int k1, int k2, const int32_t* __restrict__ 
 template <typename T>
 __global__ void __launch_bounds__(256)
 apply_perm_kernel(int64_t ncols, T* __restrict__ a, int64_t lda, const int32_t* __restrict__ pairs) {
-  constexpr int CW = 16, RPT = PERM_MAX / (256 / CW);   // 32 moves per thread
+  constexpr int CW = 16, RPT = PERM_MAX / (256 / CW);   // 64 moves per thread
   const int n = pairs[0];
   const int col = threadIdx.x % CW, q0 = threadIdx.x / CW;
   const int64_t c = blockIdx.x * (int64_t)CW + col;
