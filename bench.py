@@ -73,6 +73,36 @@ def cpu_baseline(n=2048, reps=2):
                       f"n={n} row-major uniform[-1,1], 1 RHS, best of {reps}, {best:.2f} s"}
 
 
+HBM_PEAK_GBPS = 8000.0
+
+
+def roofline_of(kernel, ms, launches, flops, bytes_, peak_tflops, steps, traffic):
+    """The binding roof of a kernel from its algorithmic work: HBM when its
+    arithmetic intensity (flops per algorithmic byte) is below the ridge
+    peak_flops / peak_bandwidth, the matrix cores otherwise. `achieved` is the
+    algorithmic rate over the summed HIP-event durations of its launches."""
+    if ms <= 0 or launches <= 0:
+        return None
+    t = ms / 1e3
+    ai = flops / bytes_ if bytes_ > 0 else float("inf")
+    ridge = peak_tflops * 1e12 / (HBM_PEAK_GBPS * 1e9)
+    out = {"kernel": kernel, "launches_per_step": launches // max(1, steps),
+           "avg_launch_ms": round(ms / launches, 4),
+           "algorithmic_flops_per_launch": round(flops / launches),
+           "algorithmic_bytes_per_launch": round(bytes_ / launches),
+           "flops_per_byte": round(ai, 1), "ridge_flops_per_byte": round(ridge, 1),
+           "traffic": traffic, "mfma_tflops": round(flops / t / 1e12, 2), "mfma_peak": peak_tflops}
+    if ai < ridge:
+        achieved = bytes_ / t / 1e9
+        out.update({"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBPS, 4)})
+    else:
+        achieved = flops / t / 1e12
+        out.update({"bound": "mfma", "achieved": round(achieved, 2), "peak": peak_tflops, "unit": "TFLOP/s",
+                    "frac": round(achieved / peak_tflops, 4)})
+    return out
+
+
 def traffic_from_profiles(name):
     path = os.path.join(ROOT, "profiles", name)
     if os.path.exists(path):
@@ -196,19 +226,9 @@ def main():
     del Aw
 
     prec = args.precision
-    achieved = (fl.value / (ms.value / 1e3) / 1e12) if ms.value > 0 else None
-    roofline = {
-        "bound": "mfma",
-        "kernel": f"mp_gemm ({prec} inputs, FP32 accumulate) trailing update",
-        "achieved": round(achieved, 2) if achieved else None,
-        "peak": PEAK[prec],
-        "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK[prec], 4) if achieved else None,
-        "traffic": traffic_from_profiles(f"traffic_mp_gemm_{prec}_n{n}.json"),
-        "launches_per_step": nl.value // max(1, args.steps),
-        "avg_launch_ms": round(ms.value / max(1, nl.value), 4),
-        "algorithmic_bytes_per_launch": round(by.value / max(1, nl.value)),
-    }
+    roofline = roofline_of(f"mp_gemm ({prec} inputs, FP32 accumulate) trailing update", ms.value, nl.value,
+                           fl.value, by.value, PEAK[prec], args.steps,
+                           traffic_from_profiles(f"traffic_mp_gemm_{prec}_n{n}.json"))
     out = {
         "metric": METRIC,
         "value": round(flops / t_mixed / 1e12, 3) if valid else None,
@@ -319,7 +339,6 @@ def bench_cholesky(args):
         del Aw
     flops = n ** 3 / 3.0
     prec = args.precision
-    achieved = (fl.value / (ms.value / 1e3) / 1e12) if ms.value > 0 else None
     out = {
         "metric": "Cholesky solve TFLOP/s at FP64 backward error, mixed+IR vs plain FP64",
         "value": round(flops / t_mixed / 1e12, 3) if be_mixed <= 16 else None,
@@ -336,11 +355,8 @@ def bench_cholesky(args):
             "backward_error": round(be_fp64, 4),
             "frac_fp64_peak": round(flops / t_fp64 / 1e12 / PEAK["fp64"], 4)},
         "speedup_vs_plain_fp64": None if t_fp64 is None else round(t_fp64 / t_mixed, 3),
-        "roofline": {"bound": "mfma", "kernel": f"mp_gemm ({prec}) lower-triangle trailing update",
-                     "achieved": round(achieved, 2) if achieved else None, "peak": PEAK[prec],
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK[prec], 4) if achieved else None,
-                     "traffic": None, "launches_per_step": nl.value // max(1, args.steps),
-                     "avg_launch_ms": round(ms.value / max(1, nl.value), 4)},
+        "roofline": roofline_of(f"mp_gemm ({prec}) lower-triangle trailing update", ms.value, nl.value, fl.value,
+                                by.value, PEAK[prec], args.steps, None),
     }
     print(json.dumps(out))
 

@@ -38,7 +38,7 @@ void gemm_f64_cm(int, int, int64_t, int64_t, int64_t, double, const double*, int
 
 namespace {
 
-constexpr int NB_MIX = 256;
+constexpr int NB_MIX = 512;   // block column of the mixed factorization (two 256-wide halves)
 
 // ---- equilibration ---------------------------------------------------------------
 __device__ __forceinline__ double pow2_scale(double amax) {
@@ -138,17 +138,19 @@ __global__ void rows_to_lowp_kernel(int64_t rows, int kb, const float* __restric
   if (lane == 0) inv[row] = 1.0f / s;
 }
 
-// YT (cols x ldy) = transpose of U (kb x cols) scaled per column; inv[j] = 1 / s_j.
-// A workgroup handles 64 columns: the kb x 64 tile goes through LDS.
+// YT (cols x ldy) = transpose of U (kb x cols, kb <= 512) scaled per column;
+// inv[j] = 1 / s_j. A workgroup handles 32 columns: the kb x 32 tile goes
+// through LDS.
 template <typename H>
-__global__ void cols_to_lowp_t_kernel(int64_t cols, int kb, const float* __restrict__ U, int64_t ldu,
-                                      H* __restrict__ YT, int64_t ldy, float* __restrict__ inv) {
-  __shared__ float tile[256][65];
-  __shared__ float cmax[4][64];
-  const int64_t j0 = blockIdx.x * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4
+__global__ void __launch_bounds__(256) cols_to_lowp_t_kernel(int64_t cols, int kb, const float* __restrict__ U,
+                                                             int64_t ldu, H* __restrict__ YT, int64_t ldy,
+                                                             float* __restrict__ inv) {
+  __shared__ float tile[512][33];
+  __shared__ float cmax[8][32];
+  const int64_t j0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
   float mx = 0.0f;
-  for (int k = ty; k < kb; k += 4) {
+  for (int k = ty; k < kb; k += 8) {
     const int64_t j = j0 + tx;
     const float v = j < cols ? U[k * ldu + j] : 0.0f;
     tile[k][tx] = v;
@@ -157,17 +159,19 @@ __global__ void cols_to_lowp_t_kernel(int64_t cols, int kb, const float* __restr
   cmax[ty][tx] = mx;
   __syncthreads();
   if (ty == 0) {
-    const float m = fmaxf(fmaxf(cmax[0][tx], cmax[1][tx]), fmaxf(cmax[2][tx], cmax[3][tx]));
+    float m = cmax[0][tx];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) m = fmaxf(m, cmax[r][tx]);
     cmax[0][tx] = pow2_scale_f(m);
     if (j0 + tx < cols) inv[j0 + tx] = 1.0f / cmax[0][tx];
   }
   __syncthreads();
-  // write row j of YT: consecutive threads take consecutive k
-  for (int jj = ty; jj < 64; jj += 4) {
+  // row j of YT: consecutive threads take consecutive k
+  for (int jj = ty; jj < 32; jj += 8) {
     const int64_t j = j0 + jj;
     if (j >= cols) break;
-    const float s = cmax[0][jj];
-    for (int k = tx; k < (int)ldy; k += 64) YT[j * ldy + k] = to_h<H>(k < kb ? tile[k][jj] * s : 0.0f);
+    const float sc = cmax[0][jj];
+    for (int k = tx; k < (int)ldy; k += 32) YT[j * ldy + k] = to_h<H>(k < kb ? tile[k][jj] * sc : 0.0f);
   }
 }
 
@@ -379,7 +383,7 @@ struct TriSolve {
   int upper;
   const float* W;            // row-major factor, ld ldw
   int64_t ldw;
-  const float* inv;          // inverses of the 256 x 256 diagonal blocks (row-major, ld = block size)
+  const float* inv;          // inverses of the NB_MIX diagonal blocks (row-major, ld = block size)
   const double* v;           // forward: y_i = fl32(rsc[perm[i]] v[perm[i]]) ...
   const int32_t* perm;
   const double* rsc;
@@ -415,12 +419,12 @@ __global__ void __launch_bounds__(256) trisolve_f32_kernel(TriSolve a) {
     const int64_t r0 = b * TS;
     const int rows = (int)std::min<int64_t>(TS, a.n - r0);
     const int wr0 = w * TS_RPW;   // first row of this wave within the block
-    // D_b = the (o, o) sub-block of the inverse of its 256-block, fetched up
+    // D_b = the (o, o) sub-block of the inverse of its NB_MIX-block, fetched up
     // front: only the last z_j is on the critical path, D_b must not be
-    const int64_t B = r0 / 256;
-    const int64_t o = r0 - B * 256;
-    const int64_t ldi = std::min<int64_t>(256, a.n - B * 256);
-    const float* D = a.inv + B * 256 * 256 + o * ldi + o;
+    const int64_t B = r0 / NB_MIX;
+    const int64_t o = r0 - B * NB_MIX;
+    const int64_t ldi = std::min<int64_t>(NB_MIX, a.n - B * NB_MIX);
+    const float* D = a.inv + B * NB_MIX * NB_MIX + o * ldi + o;
     float d0[TS_RPW], d1[TS_RPW];
 #pragma unroll
     for (int rr = 0; rr < TS_RPW; ++rr) {
@@ -570,10 +574,12 @@ struct MixedCtx {
   // optimistic panels
   bool optimistic = true;
   float tau = 16.0f;         // largest |l| accepted below the diagonal block
-  float* pbak = nullptr;     // copy of the block column (rows x DB)
-  float* uinv = nullptr;     // U11^-1 (DB x DB) and two scratch blocks
-  float* scr1 = nullptr;
+  float* pbak = nullptr;     // copy of the block column (rows x NB_MIX)
+  float* ptmp = nullptr;     // A21 of the second half (rows x DB)
+  float* scr1 = nullptr;     // DB x DB scratch blocks
   float* scr2 = nullptr;
+  float* scr3 = nullptr;
+  float* scr4 = nullptr;
   int* pfail = nullptr;
   int panels = 0, panels_fast = 0;
 };
@@ -597,7 +603,7 @@ void trailing_gemm(int64_t rows, int64_t cols, int64_t kb, const float* L, int64
   if (mx.prec == kPrecF16) {
     hipLaunchKernelGGL((rows_to_lowp_kernel<_Float16>), dim3(blocks(rows, 4)), dim3(256), 0, s, rows,
                        (int)kb, L, ldl, mx.l16, kp, mx.rs);
-    hipLaunchKernelGGL((cols_to_lowp_t_kernel<_Float16>), dim3(blocks(cols, 64)), dim3(256), 0, s, cols,
+    hipLaunchKernelGGL((cols_to_lowp_t_kernel<_Float16>), dim3(blocks(cols, 32)), dim3(256), 0, s, cols,
                        (int)kb, U, ldu, mx.u16, kp, mx.cs);
     ZH_LAUNCH_CHECK();
     void* tok = profile_begin(kProfGemmLowp, s);
@@ -608,7 +614,7 @@ void trailing_gemm(int64_t rows, int64_t cols, int64_t kb, const float* L, int64
     __bf16* u = reinterpret_cast<__bf16*>(mx.u16);
     hipLaunchKernelGGL((rows_to_lowp_kernel<__bf16>), dim3(blocks(rows, 4)), dim3(256), 0, s, rows,
                        (int)kb, L, ldl, l, kp, mx.rs);
-    hipLaunchKernelGGL((cols_to_lowp_t_kernel<__bf16>), dim3(blocks(cols, 64)), dim3(256), 0, s, cols,
+    hipLaunchKernelGGL((cols_to_lowp_t_kernel<__bf16>), dim3(blocks(cols, 32)), dim3(256), 0, s, cols,
                        (int)kb, U, ldu, u, kp, mx.cs);
     ZH_LAUNCH_CHECK();
     void* tok = profile_begin(kProfGemmLowp, s);
@@ -629,10 +635,67 @@ void trsm_f32(int64_t kb, int64_t cols, const float* L11, int64_t ld, float* B, 
   }
 }
 
-// inv <- L^-1 (kb x kb, lower; unit or not) by forward substitution on the identity
+// Inverse of a 64 x 64 lower triangular diagonal block (block b of the
+// kb x kb matrix L, unit or not) into the same block of X (ld ldx): one wave,
+// lane j solves L x = e_j by forward substitution with x in registers; the
+// factor's entries are wave-uniform loads.
+template <bool UNIT>
+__global__ void __launch_bounds__(64) tri_inv64_kernel(const float* __restrict__ L, int64_t ld,
+                                                       float* __restrict__ X, int64_t ldx) {
+  const int j = threadIdx.x;
+  const int64_t o = blockIdx.x * 64;
+  const float* Lb = L + o * ld + o;
+  float x[64];
+#pragma unroll
+  for (int k = 0; k < 64; ++k) {
+    const float* row = Lb + k * ld;
+    float acc = k == j ? 1.0f : 0.0f;
+#pragma unroll
+    for (int i = 0; i < k; ++i) acc = fmaf(-row[i], x[i], acc);
+    x[k] = UNIT ? acc : acc / row[k];
+  }
+  float* Xb = X + o * ldx + o;
+#pragma unroll
+  for (int k = 0; k < 64; ++k) Xb[k * ldx + j] = x[k];
+}
+
+// X (ld ldx) <- inverse of the 256 x 256 lower triangular L (ld ld), unit or
+// not: the four 64 x 64 diagonal inverses in one launch, then X21 = -X22 L21 X11
+// at 128 and at 256. The strictly upper part of X must already be zero. T is
+// 128 x 128 scratch.
+void tri_inv256(const float* L, int64_t ld, float* X, int64_t ldx, int unit, float* T, hipStream_t s) {
+  if (unit) hipLaunchKernelGGL(tri_inv64_kernel<true>, dim3(4), dim3(64), 0, s, L, ld, X, ldx);
+  else hipLaunchKernelGGL(tri_inv64_kernel<false>, dim3(4), dim3(64), 0, s, L, ld, X, ldx);
+  ZH_LAUNCH_CHECK();
+  for (int64_t h = 0; h < 2; ++h) {   // the two 128 x 128 diagonal blocks
+    const int64_t o = h * 128;
+    mp_gemm_f32(64, 64, 64, 1.0f, L + (o + 64) * ld + o, ld, X + o * ldx + o, ldx, 0.0f, nullptr, nullptr, T,
+                64, s);
+    mp_gemm_f32(64, 64, 64, -1.0f, X + (o + 64) * ldx + o + 64, ldx, T, 64, 0.0f, nullptr, nullptr,
+                X + (o + 64) * ldx + o, ldx, s);
+  }
+  mp_gemm_f32(128, 128, 128, 1.0f, L + 128 * ld, ld, X, ldx, 0.0f, nullptr, nullptr, T, 128, s);
+  mp_gemm_f32(128, 128, 128, -1.0f, X + 128 * ldx + 128, ldx, T, 128, 0.0f, nullptr, nullptr, X + 128 * ldx,
+              ldx, s);
+}
+
+// inv <- L^-1 (kb x kb, ld kb, lower; unit or not). kb = 256 and 512 go
+// through tri_inv256 (512: X21 = -X22 L21 X11 once more); other sizes through
+// forward substitution on the identity.
 void tri_inverse_f32(int64_t kb, const float* L, int64_t ld, float* inv, int unit, hipStream_t s) {
   hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(kb * kb)), dim3(256), 0, s, kb, inv, kb);
   ZH_LAUNCH_CHECK();
+  if (kb == 256 || kb == 512) {
+    float* T = static_cast<float*>(workspace(sizeof(float) * 256 * 256, 17));
+    tri_inv256(L, ld, inv, kb, unit, T, s);
+    if (kb == 512) {
+      tri_inv256(L + 256 * ld + 256, ld, inv + 256 * kb + 256, kb, unit, T, s);
+      mp_gemm_f32(256, 256, 256, 1.0f, L + 256 * ld, ld, inv, kb, 0.0f, nullptr, nullptr, T, 256, s);
+      mp_gemm_f32(256, 256, 256, -1.0f, inv + 256 * kb + 256, kb, T, 256, 0.0f, nullptr, nullptr, inv + 256 * kb,
+                  kb, s);
+    }
+    return;
+  }
   for (int64_t k0 = 0; k0 < kb; k0 += 64) {
     const int64_t b = std::min<int64_t>(64, kb - k0);
     // only columns < k0 + b of rows k0.. are non-zero so far: solve those
@@ -645,8 +708,44 @@ void tri_inverse_f32(int64_t kb, const float* L, int64_t ld, float* inv, int uni
   }
 }
 
-// Optimistic block column: partial pivoting inside the diagonal block, then
-// L21 = A21 U11^-1. With every |l_ij| <= 1 no row below the block would have
+void apply_inv_f32(int64_t kb, int64_t cols, const float* inv, float* B, int64_t ldb, float* tmp,
+                   hipStream_t s) {
+  ZH_CHECK(hipMemcpy2DAsync(tmp, sizeof(float) * cols, B, sizeof(float) * ldb, sizeof(float) * cols, kb,
+                            hipMemcpyDeviceToDevice, s));
+  mp_gemm_f32(kb, cols, kb, 1.0f, inv, kb, tmp, cols, 0.0f, nullptr, nullptr, B, ldb, s);
+}
+
+// One optimistic 256-wide block column: partial pivoting inside its diagonal
+// block P (ld lda, rows x DB with rows >= DB), then L21 = A21 U11^-1 with A21
+// taken from src21 (rows - DB x DB, ld src_ld; P's own rows below the block
+// are overwritten). U11^-1 goes to uinv (ld uinv_ld). *pfail records a zero
+// pivot (1) or an |l| above the threshold (2).
+void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, int64_t src_ld, int32_t* ipiv,
+                      int base, float* uinv, int64_t uinv_ld, hipStream_t s) {
+  MixedCtx& mx = tl_mix;
+  hipLaunchKernelGGL(diag_getf2_kernel, dim3(1), dim3(1024), 0, s, P, lda, ipiv, base, mx.pfail);
+  ZH_LAUNCH_CHECK();
+  // U11^-1 = ((U11^T)^-1)^T
+  hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, P, lda, mx.scr1, 1);
+  tri_inverse_f32(DB, mx.scr1, DB, mx.scr2, 0, s);
+  hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr2,
+                     (int64_t)DB, mx.scr1, 0);
+  ZH_LAUNCH_CHECK();
+  ZH_CHECK(hipMemcpy2DAsync(uinv, sizeof(float) * uinv_ld, mx.scr1, sizeof(float) * DB, sizeof(float) * DB, DB,
+                            hipMemcpyDeviceToDevice, s));
+  if (rows > DB) {
+    mp_gemm_f32(rows - DB, DB, DB, 1.0f, src21, src_ld, mx.scr1, DB, 0.0f, nullptr, nullptr, P + DB * lda, lda, s);
+    hipLaunchKernelGGL(bound_check_kernel, dim3(blocks((rows - DB) * DB / 16)), dim3(256), 0, s, rows - DB, DB,
+                       (const float*)(P + DB * lda), lda, mx.tau, mx.pfail);
+    ZH_LAUNCH_CHECK();
+  }
+}
+
+// Optimistic block column of width 256 or 512 (two 256-wide halves: the
+// first half's interchanges and U12 = L11^-1 A12 update the second half, which
+// is then factored the same way, and its interchanges reach the first half's
+// columns, as getrf2.zig:99-202 orders it). Pivots are taken inside the
+// diagonal blocks; with every |l_ij| <= 1 no row below a block would have
 // won a pivot search, so the result is the partial-pivoting one (up to
 // rounding and ties); with |l_ij| <= tau > 1 it is threshold pivoting, whose
 // growth per step is at most 1 + tau (the FP64 refinement then restores the
@@ -657,37 +756,42 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
                    hipStream_t s) {
   MixedCtx& mx = tl_mix;
   const int64_t rows = m - j0;
-  if (!mx.optimistic || jb != DB || rows < DB) return false;
+  if (!mx.optimistic || (jb != DB && jb != 2 * DB) || rows < jb) return false;
   ++mx.panels;
   float* P = A + j0 * lda + j0;
+  float* ustore = mx.invu_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;   // ld NB_MIX
   ZH_CHECK(hipMemsetAsync(mx.pfail, 0, sizeof(int), s));
-  ZH_CHECK(hipMemcpy2DAsync(mx.pbak, sizeof(float) * DB, P, sizeof(float) * lda, sizeof(float) * DB, rows,
+  ZH_CHECK(hipMemcpy2DAsync(mx.pbak, sizeof(float) * jb, P, sizeof(float) * lda, sizeof(float) * jb, rows,
                             hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(diag_getf2_kernel, dim3(1), dim3(1024), 0, s, P, lda, ipiv + j0, (int)j0, mx.pfail);
-  ZH_LAUNCH_CHECK();
-  if (rows > DB) {
-    // U11^-1 = ((U11^T)^-1)^T, then L21 = A21 U11^-1 from the saved A21
-    hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, P, lda,
-                       mx.scr1, 1);
-    tri_inverse_f32(DB, mx.scr1, DB, mx.scr2, 0, s);
-    float* uinv = mx.invu_all + (j0 / DB) * DB * DB;
-    hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr2,
-                       (int64_t)DB, uinv, 0);
-    mp_gemm_f32(rows - DB, DB, DB, 1.0f, mx.pbak + (int64_t)DB * DB, DB, uinv, DB, 0.0f, nullptr, nullptr,
-                P + DB * lda, lda, s);
-    hipLaunchKernelGGL(bound_check_kernel, dim3(blocks((rows - DB) * DB / 16)), dim3(256), 0, s, rows - DB, DB,
-                       (const float*)(P + DB * lda), lda, mx.tau, mx.pfail);
-    ZH_LAUNCH_CHECK();
+  optimistic_block(P, lda, rows, mx.pbak + DB * jb, jb, ipiv + j0, (int)j0, ustore, NB_MIX, s);
+  if (jb == 2 * DB) {
+    const int64_t c2 = j0 + DB;                 // second half: columns c2 .. c2 + DB
+    const int64_t rows2 = rows - DB;
+    const int32_t* pr = compose_perm((int)j0, (int)(c2 - 1), ipiv, s, 2);
+    apply_perm<float>(DB, A + c2, lda, pr, s);
+    // U12 of the first half = La^-1 A12, then A22 -= L21 U12 within the block column
+    tri_inverse_f32(DB, P, lda, mx.scr3, 1, s);
+    apply_inv_f32(DB, DB, mx.scr3, A + j0 * lda + c2, lda, mx.scr4, s);
+    mp_gemm_f32(rows2, DB, DB, -1.0f, A + c2 * lda + j0, lda, A + j0 * lda + c2, lda, 1.0f, nullptr, nullptr,
+                A + c2 * lda + c2, lda, s);
+    if (rows2 > DB)
+      ZH_CHECK(hipMemcpy2DAsync(mx.ptmp, sizeof(float) * DB, A + (c2 + DB) * lda + c2, sizeof(float) * lda,
+                                sizeof(float) * DB, rows2 - DB, hipMemcpyDeviceToDevice, s));
+    optimistic_block(A + c2 * lda + c2, lda, rows2, mx.ptmp, DB, ipiv + c2, (int)c2,
+                     ustore + DB * NB_MIX + DB, NB_MIX, s);
+    const int32_t* pr2 = compose_perm((int)c2, (int)(c2 + DB - 1), ipiv, s, 2);
+    apply_perm<float>(DB, A + j0, lda, pr2, s);
   }
   int fail = 0;
   ZH_CHECK(hipMemcpyAsync(&fail, mx.pfail, sizeof(int), hipMemcpyDeviceToHost, s));
   ZH_CHECK(hipStreamSynchronize(s));
   if (fail == 0) {
     ++mx.panels_fast;
-    if (rows > DB) mx.have_invu[j0 / DB] = 1;
+    // the solves read the 128 x 128 diagonal sub-blocks of U^-1: both halves are there
+    if (jb == NB_MIX) mx.have_invu[j0 / NB_MIX] = 1;
     return true;
   }
-  ZH_CHECK(hipMemcpy2DAsync(P, sizeof(float) * lda, mx.pbak, sizeof(float) * DB, sizeof(float) * DB, rows,
+  ZH_CHECK(hipMemcpy2DAsync(P, sizeof(float) * lda, mx.pbak, sizeof(float) * jb, sizeof(float) * jb, rows,
                             hipMemcpyDeviceToDevice, s));
   return false;
 }
@@ -696,12 +800,6 @@ void invert_f32(int64_t kb, const float* L11, int64_t ld, float* inv, hipStream_
   tri_inverse_f32(kb, L11, ld, inv, 1, s);
 }
 
-void apply_inv_f32(int64_t kb, int64_t cols, const float* inv, float* B, int64_t ldb, float* tmp,
-                   hipStream_t s) {
-  ZH_CHECK(hipMemcpy2DAsync(tmp, sizeof(float) * cols, B, sizeof(float) * ldb, sizeof(float) * cols, kb,
-                            hipMemcpyDeviceToDevice, s));
-  mp_gemm_f32(kb, cols, kb, 1.0f, inv, kb, tmp, cols, 0.0f, nullptr, nullptr, B, ldb, s);
-}
 
 // ---- solver state -------------------------------------------------------------------------------
 struct Solver {
@@ -1244,11 +1342,14 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
     }
   };
   {
-    char* pw = static_cast<char*>(workspace(sizeof(float) * ((size_t)(n + DB) * DB + 3 * DB * DB) + 256, 15));
-    tl_mix.pbak = reinterpret_cast<float*>(pw); pw += sizeof(float) * (size_t)(n + DB) * DB;
-    tl_mix.uinv = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
+    char* pw = static_cast<char*>(workspace(
+        sizeof(float) * ((size_t)(n + NB_MIX) * (NB_MIX + DB) + 4 * DB * DB) + 256, 15));
+    tl_mix.pbak = reinterpret_cast<float*>(pw); pw += sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX;
+    tl_mix.ptmp = reinterpret_cast<float*>(pw); pw += sizeof(float) * (size_t)(n + NB_MIX) * DB;
     tl_mix.scr1 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
     tl_mix.scr2 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
+    tl_mix.scr3 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
+    tl_mix.scr4 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
     tl_mix.pfail = reinterpret_cast<int*>(pw);
     tl_mix.panels = tl_mix.panels_fast = 0;
     tl_mix.inv_all = S.invL;

@@ -55,6 +55,12 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
                 int64_t ldb, double* X, int64_t ldx, const zmlhip_ir_opts* o, zmlhip_ir_result* res,
                 hipStream_t s);
 
+// Row interchanges of a block of rows (getrf_f64.hip): compose ipiv[k1..k2]
+// into moves, apply the moves to `ncols` columns starting at `a`.
+int32_t* compose_perm(int k1, int k2, const int32_t* ipiv, hipStream_t s, int slot);
+template <typename T>
+void apply_perm(int64_t ncols, T* a, int64_t lda, const int32_t* pairs, hipStream_t s);
+
 template <typename T>
 void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info, int64_t nb,
                      const LuOps<T>& ops, hipStream_t s);
