@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P,
                                                           int32_t* __restrict__ ipiv, int base,
                                                           int* __restrict__ fail) {
   __shared__ uint64_t wbest[16];
-  __shared__ float lcol[DB], urow[DB];
+  __shared__ float colk[DB], urow[DB];
   __shared__ int prow[DB], invp[DB], at[DB], pos[DB];
   __shared__ int bad;
   const int t = threadIdx.x, tx = t & 31, ty = t >> 5, lane = t & 63, wv = t >> 6;
@@ -260,15 +260,17 @@ __global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P,
   for (int jj = 0; jj < 8; ++jj) {
     for (int kq = 0; kq < 32; ++kq) {
       const int k = 32 * jj + kq;
-      // candidates of column k: its owners (tx == kq), rows not yet chosen
+      // the owners of column k (tx == kq) publish it and their best candidate
       uint64_t best = 0;
       if (tx == kq) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < 8; ++i) {
+          colk[ty + 32 * i] = a[i][jj];
           if (!((done >> i) & 1u)) {
             const uint64_t key = pivot_key(fabsf(a[i][jj]), ty + 32 * i);
             best = key > best ? key : best;
           }
+        }
       }
       const uint64_t other = __shfl_xor(best, 32);
       best = other > best ? other : best;
@@ -278,6 +280,8 @@ __global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P,
 #pragma unroll
       for (int w = 1; w < 16; ++w) g = wbest[w] > g ? wbest[w] : g;
       const int p = key_row(g), pt = p & 31, pi = p >> 5;
+      const float pv = colk[p];
+      const bool ok = pv != 0.0f && pv == pv;
       if (ty == pt) {   // the pivot row's owners publish the row (its value at k is the pivot)
 #pragma unroll
         for (int i = 0; i < 8; ++i)
@@ -287,28 +291,25 @@ __global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P,
           }
         done |= 1u << pi;
       }
-      if (t == 0) prow[k] = p;
-      __syncthreads();
-      const float pv = urow[k];
-      const bool ok = pv != 0.0f && pv == pv;
-      if (!ok && t == 0) bad = 1;
+      if (t == 0) {
+        prow[k] = p;
+        if (!ok) bad = 1;
+      }
+      // every thread forms its own rows' multipliers from the published column
       const bool recip = fabsf(pv) >= 0x1p-126f;
       const float rp = 1.0f / pv;
-      if (tx == kq) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          float l = 0.0f;
-          if (!((done >> i) & 1u) && ok) {
-            l = recip ? a[i][jj] * rp : a[i][jj] / pv;
-            a[i][jj] = l;
-          }
-          lcol[ty + 32 * i] = l;
-        }
-      }
-      __syncthreads();
       float li[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) li[i] = lcol[ty + 32 * i];
+      for (int i = 0; i < 8; ++i) {
+        const float c = colk[ty + 32 * i];
+        li[i] = (!((done >> i) & 1u) && ok) ? (recip ? c * rp : c / pv) : 0.0f;
+      }
+      __syncthreads();
+      if (tx == kq) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (!((done >> i) & 1u) && ok) a[i][jj] = li[i];
+      }
 #pragma unroll
       for (int j = jj; j < 8; ++j) {
         const float u = urow[tx + 32 * j];
@@ -317,8 +318,8 @@ __global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P,
           for (int i = 0; i < 8; ++i) a[i][j] -= li[i] * u;
         }
       }
-      // lcol and urow are rewritten after the next step's first barrier, wbest
-      // after its reads here: no further barrier is needed
+      // colk, urow and wbest are rewritten only after the next step's first
+      // barrier, which every thread reaches after finishing its reads here
     }
   }
   // row r of the block goes to position invp[r]; ipiv by replaying the interchanges
@@ -570,6 +571,7 @@ struct MixedCtx {
   float* inv_all = nullptr;  // L11^-1 of every block column, NB x NB each
   float* invu_all = nullptr; // U11^-1 of every block column (filled by diagonal-block panels)
   std::vector<char> have_invu;
+  float* inv_ready = nullptr; // L11^-1 the last diagonal-block panel already formed
   float* tmp = nullptr;      // inverse scratch
   // optimistic panels
   bool optimistic = true;
@@ -758,6 +760,7 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
   const int64_t rows = m - j0;
   if (!mx.optimistic || (jb != DB && jb != 2 * DB) || rows < jb) return false;
   ++mx.panels;
+  mx.inv_ready = nullptr;
   float* P = A + j0 * lda + j0;
   float* ustore = mx.invu_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;   // ld NB_MIX
   ZH_CHECK(hipMemsetAsync(mx.pfail, 0, sizeof(int), s));
@@ -769,9 +772,16 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     const int64_t rows2 = rows - DB;
     const int32_t* pr = compose_perm((int)j0, (int)(c2 - 1), ipiv, s, 2);
     apply_perm<float>(DB, A + c2, lda, pr, s);
+    // L11^-1 (512 x 512, kept for the driver's U12): La^-1 now, the rest below
+    float* linv = mx.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;
+    float* T = static_cast<float*>(workspace(sizeof(float) * 256 * 256, 17));
+    hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, linv, jb);
+    ZH_LAUNCH_CHECK();
+    tri_inv256(P, lda, linv, jb, 1, T, s);
     // U12 of the first half = La^-1 A12, then A22 -= L21 U12 within the block column
-    tri_inverse_f32(DB, P, lda, mx.scr3, 1, s);
-    apply_inv_f32(DB, DB, mx.scr3, A + j0 * lda + c2, lda, mx.scr4, s);
+    ZH_CHECK(hipMemcpy2DAsync(mx.scr4, sizeof(float) * DB, A + j0 * lda + c2, sizeof(float) * lda,
+                              sizeof(float) * DB, DB, hipMemcpyDeviceToDevice, s));
+    mp_gemm_f32(DB, DB, DB, 1.0f, linv, jb, mx.scr4, DB, 0.0f, nullptr, nullptr, A + j0 * lda + c2, lda, s);
     mp_gemm_f32(rows2, DB, DB, -1.0f, A + c2 * lda + j0, lda, A + j0 * lda + c2, lda, 1.0f, nullptr, nullptr,
                 A + c2 * lda + c2, lda, s);
     if (rows2 > DB)
@@ -781,6 +791,11 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
                      ustore + DB * NB_MIX + DB, NB_MIX, s);
     const int32_t* pr2 = compose_perm((int)c2, (int)(c2 + DB - 1), ipiv, s, 2);
     apply_perm<float>(DB, A + j0, lda, pr2, s);
+    // Lb^-1, then the off-diagonal block -Lb^-1 Lba La^-1
+    tri_inv256(A + c2 * lda + c2, lda, linv + DB * jb + DB, jb, 1, T, s);
+    mp_gemm_f32(DB, DB, DB, 1.0f, A + c2 * lda + j0, lda, linv, jb, 0.0f, nullptr, nullptr, mx.scr3, DB, s);
+    mp_gemm_f32(DB, DB, DB, -1.0f, linv + DB * jb + DB, jb, mx.scr3, DB, 0.0f, nullptr, nullptr, linv + DB * jb,
+                jb, s);
   }
   int fail = 0;
   ZH_CHECK(hipMemcpyAsync(&fail, mx.pfail, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -788,7 +803,10 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
   if (fail == 0) {
     ++mx.panels_fast;
     // the solves read the 128 x 128 diagonal sub-blocks of U^-1: both halves are there
-    if (jb == NB_MIX) mx.have_invu[j0 / NB_MIX] = 1;
+    if (jb == NB_MIX) {
+      mx.have_invu[j0 / NB_MIX] = 1;
+      mx.inv_ready = mx.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;
+    }
     return true;
   }
   ZH_CHECK(hipMemcpy2DAsync(P, sizeof(float) * lda, mx.pbak, sizeof(float) * jb, sizeof(float) * jb, rows,
@@ -1338,6 +1356,7 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   ZH_CHECK(hipMemsetAsync(S.info, 0, sizeof(int), s));
   struct Inv {
     static void store(int64_t kb, const float* L11, int64_t ld, float* inv, hipStream_t st) {
+      if (inv == tl_mix.inv_ready) return;   // formed by the diagonal-block panel
       invert_f32(kb, L11, ld, inv, st);
     }
   };
