@@ -158,6 +158,12 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
   if (mn <= 0) return;
   hipStream_t sp = aux_stream(0);
   hipEvent_t ev_in = aux_event(0), ev_panel = aux_event(1), ev_next = aux_event(2);
+  const hipStream_t caller = s;
+  if (ops.isolate_panel) {   // the trailing updates move to the CU-sparing stream
+    s = stream_sparing_cus();
+    ZH_CHECK(hipEventRecord(ev_in, caller));
+    ZH_CHECK(hipStreamWaitEvent(s, ev_in, 0));
+  }
   // L11^-1 of consecutive block columns alternate between two buffers; the
   // copy of A12 is used by the main stream only
   T* inv = static_cast<T*>(workspace(sizeof(T) * 2 * nb * nb, 0));
@@ -201,6 +207,11 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
       panel_and_inverse(c1, jb2, blk + 1);
     }
   }
+  if (s != caller) {
+    hipEvent_t ev_out = aux_event(3);
+    ZH_CHECK(hipEventRecord(ev_out, s));
+    ZH_CHECK(hipStreamWaitEvent(caller, ev_out, 0));
+  }
 }
 
 static void gemm_f64_update(int64_t rows, int64_t cols, int64_t kb, const double* L, int64_t ldl,
@@ -234,7 +245,7 @@ static void apply_inv_f64(int64_t kb, int64_t cols, const double* inv, double* B
 void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, int* info,
                   hipStream_t s) {
   const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64, nullptr, nullptr,
-                          nullptr};
+                          nullptr, false};
   getrf_lookahead<double>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }
 

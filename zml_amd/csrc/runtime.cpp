@@ -18,10 +18,12 @@ struct Buf {
 };
 constexpr int kSlots = 24;
 constexpr int kAux = 4, kEvents = 8;
+constexpr int kReservedCus = 8;   // CUs a sparing stream leaves to the others
 struct DeviceState {
   Buf slot[kSlots];
   hipStream_t stream = nullptr;
   hipStream_t aux[kAux] = {};
+  hipStream_t reserved = nullptr;   // every CU but the first kReservedCus
   hipEvent_t ev[kEvents] = {};
 };
 struct ThreadState {
@@ -36,6 +38,7 @@ struct ThreadState {
       if (kv.second.stream) (void)hipStreamDestroy(kv.second.stream);
       for (auto& st : kv.second.aux)
         if (st) (void)hipStreamDestroy(st);
+      if (kv.second.reserved) (void)hipStreamDestroy(kv.second.reserved);
       for (auto& e : kv.second.ev)
         if (e) (void)hipEventDestroy(e);
       (void)hipSetDevice(cur);
@@ -134,6 +137,17 @@ hipStream_t aux_stream(int idx) {
   hipStream_t& st = ds.aux[idx % kAux];
   if (!st) ZH_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   return st;
+}
+
+hipStream_t stream_sparing_cus() {
+  DeviceState& ds = current();
+  if (!ds.reserved) {
+    const int cus = device_cu_count();
+    std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+    for (int c = kReservedCus; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+    ZH_CHECK(hipExtStreamCreateWithCUMask(&ds.reserved, (uint32_t)mask.size(), mask.data()));
+  }
+  return ds.reserved;
 }
 
 hipEvent_t aux_event(int idx) {
