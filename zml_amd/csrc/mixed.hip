@@ -661,24 +661,129 @@ __global__ void __launch_bounds__(64) tri_inv64_kernel(const float* __restrict__
   for (int k = 0; k < 64; ++k) Xb[k * ldx + j] = x[k];
 }
 
-// X (ld ldx) <- inverse of the 256 x 256 lower triangular L (ld ld), unit or
-// not: the four 64 x 64 diagonal inverses in one launch, then X21 = -X22 L21 X11
-// at 128 and at 256. The strictly upper part of X must already be zero. T is
-// 128 x 128 scratch.
-void tri_inv256(const float* L, int64_t ld, float* X, int64_t ldx, int unit, float* T, hipStream_t s) {
-  if (unit) hipLaunchKernelGGL(tri_inv64_kernel<true>, dim3(4), dim3(64), 0, s, L, ld, X, ldx);
-  else hipLaunchKernelGGL(tri_inv64_kernel<false>, dim3(4), dim3(64), 0, s, L, ld, X, ldx);
-  ZH_LAUNCH_CHECK();
-  for (int64_t h = 0; h < 2; ++h) {   // the two 128 x 128 diagonal blocks
-    const int64_t o = h * 128;
-    mp_gemm_f32(64, 64, 64, 1.0f, L + (o + 64) * ld + o, ld, X + o * ldx + o, ldx, 0.0f, nullptr, nullptr, T,
-                64, s);
-    mp_gemm_f32(64, 64, 64, -1.0f, X + (o + 64) * ldx + o + 64, ldx, T, 64, 0.0f, nullptr, nullptr,
-                X + (o + 64) * ldx + o, ldx, s);
+// X (ld ldx) <- inverse of a 256 x 256 triangular matrix in one workgroup.
+// !TRANS: L lower (ld ld), X = L^-1. TRANS: the input is an upper U; the
+// kernel inverts L = U^T (reading U transposed) and writes X^T = U^-1.
+// Phase A: wave w inverts the 64 x 64 diagonal block w (lane j solves for
+// column j, the block staged in LDS). Phase B, column block J = 0..2 and row
+// block I = J+1..3 in order: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ, as 64 x 64
+// products over LDS (thread: 4 x 4 outputs). Only X's lower blocks are
+// written; the caller zeroes the rest.
+constexpr int TI_LD = 65;
+template <bool UNIT, bool TRANS>
+__global__ void __launch_bounds__(256) tri_inv256_kernel(const float* __restrict__ L, int64_t ld,
+                                                         float* __restrict__ X, int64_t ldx) {
+  __shared__ float smem[9 * 64 * TI_LD];
+  float* Xd = smem;                       // 4 diagonal inverses, 64 x TI_LD each
+  float* Xo = Xd + 4 * 64 * TI_LD;        // this column block's X_KJ, K > J (3)
+  float* Ls = Xo + 3 * 64 * TI_LD;        // L_IK staging
+  float* Tm = Ls + 64 * TI_LD;            // sum of products
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  // element (r, c) of the lower factor, block (I, K)
+  auto lget = [&](int I, int K, int r, int c) -> float {
+    const int64_t gr = I * 64 + r, gc = K * 64 + c;
+    return TRANS ? L[gc * ld + gr] : L[gr * ld + gc];
+  };
+  auto xput = [&](int I, int J, int r, int c, float v) {
+    const int64_t gr = I * 64 + r, gc = J * 64 + c;
+    if (TRANS) X[gc * ldx + gr] = v;
+    else X[gr * ldx + gc] = v;
+  };
+  // ---- phase A (Xo is free yet: it stages the four diagonal blocks)
+  {
+    float* st = Xo + w * 64 * TI_LD;   // 4 blocks fit in Xo + Ls
+    for (int e = lane; e < 64 * 64; e += 64) st[(e >> 6) * TI_LD + (e & 63)] = lget(w, w, e >> 6, e & 63);
+    __syncthreads();
+    float x[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+      const float* row = st + k * TI_LD;
+      float acc = k == lane ? 1.0f : 0.0f;
+#pragma unroll
+      for (int i = 0; i < k; ++i) acc = fmaf(-row[i], x[i], acc);
+      x[k] = UNIT ? acc : acc / row[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+      Xd[(w * 64 + k) * TI_LD + lane] = x[k];
+      xput(w, w, k, lane, x[k]);
+    }
   }
-  mp_gemm_f32(128, 128, 128, 1.0f, L + 128 * ld, ld, X, ldx, 0.0f, nullptr, nullptr, T, 128, s);
-  mp_gemm_f32(128, 128, 128, -1.0f, X + 128 * ldx + 128, ldx, T, 128, 0.0f, nullptr, nullptr, X + 128 * ldx,
-              ldx, s);
+  __syncthreads();
+  // ---- phase B
+  const int r0 = (t >> 4) * 4, c0 = (t & 15) * 4;
+  for (int J = 0; J < 3; ++J) {
+    for (int I = J + 1; I < 4; ++I) {
+      float acc[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+      for (int K = J; K < I; ++K) {
+        for (int e = t; e < 64 * 64; e += 256) Ls[(e >> 6) * TI_LD + (e & 63)] = lget(I, K, e >> 6, e & 63);
+        __syncthreads();
+        const float* Q = K == J ? Xd + J * 64 * TI_LD : Xo + (K - J - 1) * 64 * TI_LD;
+#pragma unroll 8
+        for (int k = 0; k < 64; ++k) {
+          float pa[4], qb[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) pa[a] = Ls[(r0 + a) * TI_LD + k];
+#pragma unroll
+          for (int b = 0; b < 4; ++b) qb[b] = Q[k * TI_LD + c0 + b];
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(pa[a], qb[b], acc[a][b]);
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) Tm[(r0 + a) * TI_LD + c0 + b] = acc[a][b];
+      __syncthreads();
+      const float* D = Xd + I * 64 * TI_LD;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+#pragma unroll 8
+      for (int k = 0; k < 64; ++k) {
+        float pa[4], qb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) pa[a] = D[(r0 + a) * TI_LD + k];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) qb[b] = Tm[k * TI_LD + c0 + b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(pa[a], qb[b], acc[a][b]);
+      }
+      float* O = Xo + (I - J - 1) * 64 * TI_LD;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          O[(r0 + a) * TI_LD + c0 + b] = -acc[a][b];
+          xput(I, J, r0 + a, c0 + b, -acc[a][b]);
+        }
+      __syncthreads();
+    }
+  }
+}
+
+// X (ld ldx) <- inverse of the 256 x 256 lower triangular L (ld ld), unit or
+// not (trans: of the upper U, through U^T). The strictly upper part of X must
+// already be zero (lower part for trans).
+void tri_inv256(const float* L, int64_t ld, float* X, int64_t ldx, int unit, hipStream_t s, bool trans = false) {
+  if (trans) {
+    if (unit) hipLaunchKernelGGL((tri_inv256_kernel<true, true>), dim3(1), dim3(256), 0, s, L, ld, X, ldx);
+    else hipLaunchKernelGGL((tri_inv256_kernel<false, true>), dim3(1), dim3(256), 0, s, L, ld, X, ldx);
+  } else {
+    if (unit) hipLaunchKernelGGL((tri_inv256_kernel<true, false>), dim3(1), dim3(256), 0, s, L, ld, X, ldx);
+    else hipLaunchKernelGGL((tri_inv256_kernel<false, false>), dim3(1), dim3(256), 0, s, L, ld, X, ldx);
+  }
+  ZH_LAUNCH_CHECK();
 }
 
 // inv <- L^-1 (kb x kb, ld kb, lower; unit or not). kb = 256 and 512 go
@@ -689,9 +794,9 @@ void tri_inverse_f32(int64_t kb, const float* L, int64_t ld, float* inv, int uni
   ZH_LAUNCH_CHECK();
   if (kb == 256 || kb == 512) {
     float* T = static_cast<float*>(workspace(sizeof(float) * 256 * 256, 17));
-    tri_inv256(L, ld, inv, kb, unit, T, s);
+    tri_inv256(L, ld, inv, kb, unit, s);
     if (kb == 512) {
-      tri_inv256(L + 256 * ld + 256, ld, inv + 256 * kb + 256, kb, unit, T, s);
+      tri_inv256(L + 256 * ld + 256, ld, inv + 256 * kb + 256, kb, unit, s);
       mp_gemm_f32(256, 256, 256, 1.0f, L + 256 * ld, ld, inv, kb, 0.0f, nullptr, nullptr, T, 256, s);
       mp_gemm_f32(256, 256, 256, -1.0f, inv + 256 * kb + 256, kb, T, 256, 0.0f, nullptr, nullptr, inv + 256 * kb,
                   kb, s);
@@ -727,12 +832,11 @@ void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, i
   MixedCtx& mx = tl_mix;
   hipLaunchKernelGGL(diag_getf2_kernel, dim3(1), dim3(1024), 0, s, P, lda, ipiv, base, mx.pfail);
   ZH_LAUNCH_CHECK();
-  // U11^-1 = ((U11^T)^-1)^T
-  hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, P, lda, mx.scr1, 1);
-  tri_inverse_f32(DB, mx.scr1, DB, mx.scr2, 0, s);
-  hipLaunchKernelGGL(transpose_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr2,
-                     (int64_t)DB, mx.scr1, 0);
+  // U11^-1 (upper) into scr1, whose strictly lower part must be zero
+  hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr1,
+                     (int64_t)DB);
   ZH_LAUNCH_CHECK();
+  tri_inv256(P, lda, mx.scr1, DB, 0, s, true);
   ZH_CHECK(hipMemcpy2DAsync(uinv, sizeof(float) * uinv_ld, mx.scr1, sizeof(float) * DB, sizeof(float) * DB, DB,
                             hipMemcpyDeviceToDevice, s));
   if (rows > DB) {
@@ -774,10 +878,9 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     apply_perm<float>(DB, A + c2, lda, pr, s);
     // L11^-1 (512 x 512, kept for the driver's U12): La^-1 now, the rest below
     float* linv = mx.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;
-    float* T = static_cast<float*>(workspace(sizeof(float) * 256 * 256, 17));
     hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, linv, jb);
     ZH_LAUNCH_CHECK();
-    tri_inv256(P, lda, linv, jb, 1, T, s);
+    tri_inv256(P, lda, linv, jb, 1, s);
     // U12 of the first half = La^-1 A12, then A22 -= L21 U12 within the block column
     ZH_CHECK(hipMemcpy2DAsync(mx.scr4, sizeof(float) * DB, A + j0 * lda + c2, sizeof(float) * lda,
                               sizeof(float) * DB, DB, hipMemcpyDeviceToDevice, s));
@@ -792,7 +895,7 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     const int32_t* pr2 = compose_perm((int)c2, (int)(c2 + DB - 1), ipiv, s, 2);
     apply_perm<float>(DB, A + j0, lda, pr2, s);
     // Lb^-1, then the off-diagonal block -Lb^-1 Lba La^-1
-    tri_inv256(A + c2 * lda + c2, lda, linv + DB * jb + DB, jb, 1, T, s);
+    tri_inv256(A + c2 * lda + c2, lda, linv + DB * jb + DB, jb, 1, s);
     mp_gemm_f32(DB, DB, DB, 1.0f, A + c2 * lda + j0, lda, linv, jb, 0.0f, nullptr, nullptr, mx.scr3, DB, s);
     mp_gemm_f32(DB, DB, DB, -1.0f, linv + DB * jb + DB, jb, mx.scr3, DB, 0.0f, nullptr, nullptr, linv + DB * jb,
                 jb, s);
