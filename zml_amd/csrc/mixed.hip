@@ -221,8 +221,9 @@ __global__ void transpose_f32_kernel(int64_t kb, const float* __restrict__ src, 
 
 // ---- optimistic panel: pivots confined to the diagonal block ----------------------------
 // Partial-pivoting LU of a 256 x 256 FP32 block held in the registers of one
-// 1024-thread workgroup: thread (ty, tx) of a 32 x 32 grid holds rows
-// ty + 32 i and columns tx + 32 j (i, j < 8). Row interchanges are virtual: a
+// workgroup: thread (ty, tx) of a TY x 32 grid holds rows ty + TY i (i < 256 / TY)
+// and columns tx + 32 j (j < 8). Fewer, fuller threads cut the per-column
+// bookkeeping every wave repeats (pivot keys, multipliers). Row interchanges are virtual: a
 // chosen pivot row is marked done and the elimination simply skips done rows;
 // the block is written back in pivot order at the end, and ipiv gets the
 // LAPACK form (sequential interchanges, 1-based, offset by `base`). Pivot =
@@ -230,6 +231,7 @@ __global__ void transpose_f32_kernel(int64_t kb, const float* __restrict__ src, 
 // block. l = a * fl(1/p) when |p| >= FLT_MIN, else a / p (the rule of zml's
 // getrf2.zig:65-87). A zero or NaN pivot sets *fail.
 constexpr int DB = 256;
+constexpr int DIAG_TY = 32;   // thread rows of the diagonal-block LU (32 x 32 threads measured fastest)
 
 __device__ __forceinline__ uint64_t pivot_key(float v, int r) {
   // larger |a| first, then lower row; NaN below every number, 0 = no candidate
@@ -241,33 +243,35 @@ __device__ __forceinline__ int key_row(uint64_t k) {
   return lo >= 0x80000000u ? (int)(0xffffffffu - lo) : (int)(0x7fffffffu - lo);
 }
 
-__global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P, int64_t ld,
-                                                          int32_t* __restrict__ ipiv, int base,
-                                                          int* __restrict__ fail) {
-  __shared__ uint64_t wbest[16];
+template <int TY>
+__global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__ P, int64_t ld,
+                                                            int32_t* __restrict__ ipiv, int base,
+                                                            int* __restrict__ fail) {
+  constexpr int TX = 32, RI = DB / TY, RJ = DB / TX, NT = TY * TX, NWV = NT / 64;
+  __shared__ uint64_t wbest[NWV];
   __shared__ float colk[DB], urow[DB];
   __shared__ int prow[DB], invp[DB], at[DB], pos[DB];
   __shared__ int bad;
-  const int t = threadIdx.x, tx = t & 31, ty = t >> 5, lane = t & 63, wv = t >> 6;
-  float a[8][8];
+  const int t = threadIdx.x, tx = t % TX, ty = t / TX, lane = t & 63, wv = t >> 6;
+  float a[RI][RJ];
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < RI; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) a[i][j] = P[(int64_t)(ty + 32 * i) * ld + tx + 32 * j];
+    for (int j = 0; j < RJ; ++j) a[i][j] = P[(int64_t)(ty + TY * i) * ld + tx + TX * j];
   if (t == 0) bad = 0;
   unsigned done = 0;
 #pragma unroll
-  for (int jj = 0; jj < 8; ++jj) {
-    for (int kq = 0; kq < 32; ++kq) {
-      const int k = 32 * jj + kq;
+  for (int jj = 0; jj < RJ; ++jj) {
+    for (int kq = 0; kq < TX; ++kq) {
+      const int k = TX * jj + kq;
       // the owners of column k (tx == kq) publish it and their best candidate
       uint64_t best = 0;
       if (tx == kq) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          colk[ty + 32 * i] = a[i][jj];
+        for (int i = 0; i < RI; ++i) {
+          colk[ty + TY * i] = a[i][jj];
           if (!((done >> i) & 1u)) {
-            const uint64_t key = pivot_key(fabsf(a[i][jj]), ty + 32 * i);
+            const uint64_t key = pivot_key(fabsf(a[i][jj]), ty + TY * i);
             best = key > best ? key : best;
           }
         }
@@ -278,16 +282,16 @@ __global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P,
       __syncthreads();
       uint64_t g = wbest[0];
 #pragma unroll
-      for (int w = 1; w < 16; ++w) g = wbest[w] > g ? wbest[w] : g;
-      const int p = key_row(g), pt = p & 31, pi = p >> 5;
+      for (int w = 1; w < NWV; ++w) g = wbest[w] > g ? wbest[w] : g;
+      const int p = key_row(g), pt = p % TY, pi = p / TY;
       const float pv = colk[p];
       const bool ok = pv != 0.0f && pv == pv;
       if (ty == pt) {   // the pivot row's owners publish the row (its value at k is the pivot)
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < RI; ++i)
           if (i == pi) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) urow[tx + 32 * j] = a[i][j];
+            for (int j = 0; j < RJ; ++j) urow[tx + TX * j] = a[i][j];
           }
         done |= 1u << pi;
       }
@@ -298,24 +302,24 @@ __global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P,
       // every thread forms its own rows' multipliers from the published column
       const bool recip = fabsf(pv) >= 0x1p-126f;
       const float rp = 1.0f / pv;
-      float li[8];
+      float li[RI];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const float c = colk[ty + 32 * i];
+      for (int i = 0; i < RI; ++i) {
+        const float c = colk[ty + TY * i];
         li[i] = (!((done >> i) & 1u) && ok) ? (recip ? c * rp : c / pv) : 0.0f;
       }
       __syncthreads();
       if (tx == kq) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < RI; ++i)
           if (!((done >> i) & 1u) && ok) a[i][jj] = li[i];
       }
 #pragma unroll
-      for (int j = jj; j < 8; ++j) {
-        const float u = urow[tx + 32 * j];
+      for (int j = jj; j < RJ; ++j) {
+        const float u = urow[tx + TX * j];
         if (j > jj || tx > kq) {
 #pragma unroll
-          for (int i = 0; i < 8; ++i) a[i][j] -= li[i] * u;
+          for (int i = 0; i < RI; ++i) a[i][j] -= li[i] * u;
         }
       }
       // colk, urow and wbest are rewritten only after the next step's first
@@ -341,10 +345,10 @@ __global__ void __launch_bounds__(1024) diag_getf2_kernel(float* __restrict__ P,
     if (bad) atomicOr(fail, 1);
   }
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int64_t row = invp[ty + 32 * i];
+  for (int i = 0; i < RI; ++i) {
+    const int64_t row = invp[ty + TY * i];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) P[row * ld + tx + 32 * j] = a[i][j];
+    for (int j = 0; j < RJ; ++j) P[row * ld + tx + TX * j] = a[i][j];
   }
 }
 
@@ -830,7 +834,7 @@ void apply_inv_f32(int64_t kb, int64_t cols, const float* inv, float* B, int64_t
 void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, int64_t src_ld, int32_t* ipiv,
                       int base, float* uinv, int64_t uinv_ld, hipStream_t s) {
   MixedCtx& mx = tl_mix;
-  hipLaunchKernelGGL(diag_getf2_kernel, dim3(1), dim3(1024), 0, s, P, lda, ipiv, base, mx.pfail);
+  hipLaunchKernelGGL(diag_getf2_kernel<DIAG_TY>, dim3(1), dim3(DIAG_TY * 32), 0, s, P, lda, ipiv, base, mx.pfail);
   ZH_LAUNCH_CHECK();
   // U11^-1 (upper) into scr1, whose strictly lower part must be zero
   hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr1,
