@@ -1215,13 +1215,13 @@ __global__ void unpack_add_masked_kernel(int64_t n, int nr, const float* __restr
   if ((mask >> q) & 1u) D[q * ldd + i] += d[i] * (double)Y[e];
 }
 
-// C (k0 x nr, ld nr) -= A^T B, A (kb x k0, row-major, ld lda), B (kb x nr, ld nr), nr <= 16:
+// C (k0 x nr, ld nr) -= A^T B, A (kb x k0, row-major, ld lda), B (kb x nr, ld nr), kb <= NB_MIX, nr <= 16:
 // one thread per row of C, B staged in LDS
 constexpr int MAXR = 16;
 __global__ void __launch_bounds__(256) atb_sub_kernel(int64_t k0, int kb, int nr, const float* __restrict__ A,
                                                       int64_t lda, const float* __restrict__ B,
                                                       float* __restrict__ C) {
-  __shared__ float Bs[256 * MAXR];
+  __shared__ float Bs[NB_MIX * MAXR];   // kb <= NB_MIX (checked by the caller)
   for (int e = threadIdx.x; e < kb * nr; e += blockDim.x) Bs[e] = B[e];
   __syncthreads();
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1255,69 +1255,110 @@ struct CholWork {
   double* inv64 = nullptr;     // its inverse
   int* info_blk = nullptr;     // potrf info of every diagonal block
   float* tmp = nullptr;        // A21 copy, (n + NB) x NB
-  float* l21t = nullptr;       // L21^T for the FP32 update, NB x (n + NB)
-  _Float16* l16 = nullptr;     // 16-bit L21, (n + NB) x NB
-  float* rs = nullptr;         // its row scales
+  float* l21t[2] = {};         // L21^T for the FP32 update, NB x (n + NB), per block parity
+  _Float16* l16[2] = {};       // 16-bit L21, (n + NB) x NB, per block parity
+  float* rs[2] = {};           // its row scales
 };
 
 // Lower Cholesky of W + shift I (W row-major, the lower triangle): diagonal
 // blocks in FP64 (potrf and their inverses), L21 = A21 L11^-T as an FP32 GEMM
 // with that inverse, the trailing lower triangle updated in FP32 / FP16 / BF16.
+// Look-ahead of depth one: the panel stream factors block b+1 (and rounds its
+// L21 to 16 bits) while the main stream, which spares a few CUs for it,
+// applies block b beyond block b+1.
 void chol_factor(Solver& S, const CholWork& cw, int prec, double shift) {
   const int64_t n = S.n;
-  hipStream_t s = S.s;
-  ZH_CHECK(hipMemsetAsync(cw.info_blk, 0, sizeof(int) * S.nblk, s));
-  for (int64_t j0 = 0, b = 0; j0 < n; j0 += NB_MIX, ++b) {
+  const hipStream_t caller = S.s;
+  const hipStream_t sm = stream_sparing_cus(), sp = aux_stream(0);
+  hipEvent_t ev_in = aux_event(0), ev_panel = aux_event(1), ev_next = aux_event(2), ev_out = aux_event(3);
+  ZH_CHECK(hipMemsetAsync(cw.info_blk, 0, sizeof(int) * S.nblk, caller));
+  ZH_CHECK(hipEventRecord(ev_in, caller));
+  ZH_CHECK(hipStreamWaitEvent(sm, ev_in, 0));
+  ZH_CHECK(hipStreamWaitEvent(sp, ev_in, 0));
+  const int64_t kp = NB_MIX;
+  // block column b: diagonal factor, inverse, L21, and L21's update operands
+  auto panel = [&](int64_t j0, int64_t b) {
     const int64_t jb = std::min<int64_t>(NB_MIX, n - j0), m2 = n - j0 - jb;
+    const int par = (int)(b & 1);
     float* Wd = S.W + j0 * n + j0;
-    hipLaunchKernelGGL(block_to_f64_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, (const float*)Wd, n,
+    hipLaunchKernelGGL(block_to_f64_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb, (const float*)Wd, n,
                        cw.d64, jb, shift);
     ZH_LAUNCH_CHECK();
     // row-major lower = column-major upper of the same memory
-    potrf_f64_cm(kUpper, jb, cw.d64, jb, cw.info_blk + b, s);
-    hipLaunchKernelGGL(fill_identity_f64_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, cw.inv64);
+    potrf_f64_cm(kUpper, jb, cw.d64, jb, cw.info_blk + b, sp);
+    hipLaunchKernelGGL(fill_identity_f64_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb, cw.inv64);
     ZH_LAUNCH_CHECK();
     // (row-major) L11 X = I  <=>  (column-major) X' U = I with U the column-major view of L11
-    trsm_f64_cm(kRight, kUpper, kNoTrans, kNonUnit, jb, jb, 1.0, cw.d64, jb, cw.inv64, jb, s);
+    trsm_f64_cm(kRight, kUpper, kNoTrans, kNonUnit, jb, jb, 1.0, cw.d64, jb, cw.inv64, jb, sp);
     float* invL = S.invL + b * NB_MIX * NB_MIX;
     float* invLT = S.invLT + b * NB_MIX * NB_MIX;
-    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, (const double*)cw.d64, jb,
+    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb, (const double*)cw.d64, jb,
                        Wd, n, 0, 1);
-    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb,
+    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb,
                        (const double*)cw.inv64, jb, invL, jb, 0, 0);
-    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb,
+    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb,
                        (const double*)cw.inv64, jb, invLT, jb, 1, 0);
     ZH_LAUNCH_CHECK();
-    if (m2 <= 0) continue;
-    float* A21 = S.W + (j0 + jb) * n + j0;
-    ZH_CHECK(hipMemcpy2DAsync(cw.tmp, sizeof(float) * jb, A21, sizeof(float) * n, sizeof(float) * jb, m2,
-                              hipMemcpyDeviceToDevice, s));
-    mp_gemm_f32(m2, jb, jb, 1.0f, cw.tmp, jb, invLT, jb, 0.0f, nullptr, nullptr, A21, n, s);
-    float* C = S.W + (j0 + jb) * n + j0 + jb;
-    void* tok = profile_begin(kProfGemmLowp, s);
-    if (prec == kPrecF32) {
-      hipLaunchKernelGGL(transpose_rect_f32_kernel, dim3(blocks(jb, 32), blocks(m2, 32)), dim3(32, 8), 0, s, m2, jb,
-                         (const float*)A21, n, cw.l21t, m2);
-      ZH_LAUNCH_CHECK();
-      mp_gemm_f32(m2, m2, jb, -1.0f, A21, n, cw.l21t, m2, 1.0f, nullptr, nullptr, C, n, s, 1);
-    } else {
-      const int64_t kp = NB_MIX;
-      if (prec == kPrecF16) {
-        hipLaunchKernelGGL((rows_to_lowp_kernel<_Float16>), dim3(blocks(m2, 4)), dim3(256), 0, s, m2, (int)jb,
-                           (const float*)A21, n, cw.l16, kp, cw.rs);
-        ZH_LAUNCH_CHECK();
-        mp_gemm_f16(m2, m2, jb, -1.0f, cw.l16, kp, cw.l16, kp, 1.0f, cw.rs, cw.rs, C, n, s, 1);
+    if (m2 > 0) {
+      float* A21 = S.W + (j0 + jb) * n + j0;
+      ZH_CHECK(hipMemcpy2DAsync(cw.tmp, sizeof(float) * jb, A21, sizeof(float) * n, sizeof(float) * jb, m2,
+                                hipMemcpyDeviceToDevice, sp));
+      mp_gemm_f32(m2, jb, jb, 1.0f, cw.tmp, jb, invLT, jb, 0.0f, nullptr, nullptr, A21, n, sp);
+      if (prec == kPrecF32) {
+        hipLaunchKernelGGL(transpose_rect_f32_kernel, dim3(blocks(jb, 32), blocks(m2, 32)), dim3(32, 8), 0, sp, m2,
+                           jb, (const float*)A21, n, cw.l21t[par], m2);
+      } else if (prec == kPrecF16) {
+        hipLaunchKernelGGL((rows_to_lowp_kernel<_Float16>), dim3(blocks(m2, 4)), dim3(256), 0, sp, m2, (int)jb,
+                           (const float*)A21, n, cw.l16[par], kp, cw.rs[par]);
       } else {
-        __bf16* l = reinterpret_cast<__bf16*>(cw.l16);
-        hipLaunchKernelGGL((rows_to_lowp_kernel<__bf16>), dim3(blocks(m2, 4)), dim3(256), 0, s, m2, (int)jb,
-                           (const float*)A21, n, l, kp, cw.rs);
-        ZH_LAUNCH_CHECK();
-        mp_gemm_bf16(m2, m2, jb, -1.0f, l, kp, l, kp, 1.0f, cw.rs, cw.rs, C, n, s, 1);
+        hipLaunchKernelGGL((rows_to_lowp_kernel<__bf16>), dim3(blocks(m2, 4)), dim3(256), 0, sp, m2, (int)jb,
+                           (const float*)A21, n, reinterpret_cast<__bf16*>(cw.l16[par]), kp, cw.rs[par]);
       }
+      ZH_LAUNCH_CHECK();
     }
-    // lower triangle only: m2 (m2 + 1) jb flops
-    profile_end(tok, s, (double)m2 * (m2 + 1) * jb, 4.0 * (double)m2 * (m2 + 1) + 2.0 * m2 * jb);
+    ZH_CHECK(hipEventRecord(ev_panel, sp));
+  };
+  // C (rows r0.., cols c0.., lower when `lower`) -= L21[r] L21[c]^T of block b
+  auto update = [&](int64_t j0, int64_t b, int64_t r0, int64_t c0, int64_t rows, int64_t cols, int lower) {
+    const int64_t jb = std::min<int64_t>(NB_MIX, n - j0), base = j0 + jb;
+    const int par = (int)(b & 1);
+    float* C = S.W + r0 * n + c0;
+    const int64_t ro = r0 - base, co = c0 - base;   // offsets into the block's L21 rows
+    void* tok = profile_begin(kProfGemmLowp, sm);
+    if (prec == kPrecF32) {
+      mp_gemm_f32(rows, cols, jb, -1.0f, S.W + r0 * n + j0, n, cw.l21t[par] + co, n - base, 1.0f, nullptr, nullptr,
+                  C, n, sm, lower);
+    } else if (prec == kPrecF16) {
+      mp_gemm_f16(rows, cols, jb, -1.0f, cw.l16[par] + ro * kp, kp, cw.l16[par] + co * kp, kp, 1.0f,
+                  cw.rs[par] + ro, cw.rs[par] + co, C, n, sm, lower);
+    } else {
+      const __bf16* l = reinterpret_cast<const __bf16*>(cw.l16[par]);
+      mp_gemm_bf16(rows, cols, jb, -1.0f, l + ro * kp, kp, l + co * kp, kp, 1.0f, cw.rs[par] + ro, cw.rs[par] + co,
+                   C, n, sm, lower);
+    }
+    const double fl = lower ? (double)rows * (rows + 1) * jb : 2.0 * rows * cols * jb;
+    const double by = lower ? 4.0 * (double)rows * (rows + 1) + 2.0 * rows * jb : 8.0 * rows * cols + 2.0 * jb * (rows + cols);
+    profile_end(tok, sm, fl, by);
+  };
+  panel(0, 0);
+  for (int64_t j0 = 0, b = 0; j0 < n; j0 += NB_MIX, ++b) {
+    const int64_t jb = std::min<int64_t>(NB_MIX, n - j0);
+    const int64_t c1 = j0 + jb, m2 = n - c1;
+    if (m2 <= 0) break;
+    const int64_t jb2 = std::min<int64_t>(NB_MIX, m2);
+    ZH_CHECK(hipStreamWaitEvent(sm, ev_panel, 0));
+    // the next block column first (all its rows), then hand it to the panel stream
+    update(j0, b, c1, c1, m2, jb2, 0);
+    ZH_CHECK(hipEventRecord(ev_next, sm));
+    // the rest of the lower triangle, beyond the next block column
+    if (m2 > jb2) update(j0, b, c1 + jb2, c1 + jb2, m2 - jb2, m2 - jb2, 1);
+    ZH_CHECK(hipStreamWaitEvent(sp, ev_next, 0));
+    panel(c1, b + 1);
   }
+  ZH_CHECK(hipEventRecord(ev_out, sp));
+  ZH_CHECK(hipStreamWaitEvent(caller, ev_out, 0));
+  ZH_CHECK(hipEventRecord(ev_out, sm));
+  ZH_CHECK(hipStreamWaitEvent(caller, ev_out, 0));
 }
 
 // Y (n x nr, FP32 row-major, ld nr) <- L^-T L^-1 Y with the blocked factor in S.W
@@ -1339,6 +1380,7 @@ void chol_trsm_block(const Solver& S, int nr, float* Y) {
                 S.tbuf, nr, s);
     ZH_CHECK(hipMemcpyAsync(Y + k0 * nr, S.tbuf, sizeof(float) * kb * nr, hipMemcpyDeviceToDevice, s));
     if (k0 > 0) {
+      if (kb > NB_MIX || nr > MAXR) throw std::logic_error("chol_trsm_block: block too large");
       hipLaunchKernelGGL(atb_sub_kernel, dim3(blocks(k0)), dim3(256), 0, s, k0, (int)kb, nr,
                          (const float*)(S.W + k0 * n), n, (const float*)(Y + k0 * nr), Y);
       ZH_LAUNCH_CHECK();
@@ -1659,13 +1701,16 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
   cw.inv64 = reinterpret_cast<double*>(carve(sizeof(double) * blk2));
   cw.info_blk = reinterpret_cast<int*>(carve(sizeof(int) * (S.nblk + 64)));
   double* cmax = reinterpret_cast<double*>(carve(sizeof(double) * 2 * MAXR));
-  char* lp = static_cast<char*>(workspace(sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX * 2 +
-                                              sizeof(float) * (n + NB_MIX) + 4096, 10));
+  char* lp = static_cast<char*>(workspace(sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX * 3 +
+                                              sizeof(float) * 2 * (n + NB_MIX) + 4096, 10));
   cw.tmp = reinterpret_cast<float*>(lp); lp += sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX;
-  cw.l21t = reinterpret_cast<float*>(lp);   // the FP32 update uses l21t, the 16-bit ones l16 + rs
-  cw.l16 = reinterpret_cast<_Float16*>(lp);
-  lp += sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX;
-  cw.rs = reinterpret_cast<float*>(lp);
+  for (int q = 0; q < 2; ++q) {   // the FP32 update uses l21t, the 16-bit ones l16 + rs
+    cw.l21t[q] = reinterpret_cast<float*>(lp);
+    cw.l16[q] = reinterpret_cast<_Float16*>(lp);
+    lp += sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX;
+    cw.rs[q] = reinterpret_cast<float*>(lp);
+    lp += sizeof(float) * (n + NB_MIX);
+  }
 
   // ---- 1. symmetric FP64 copy, norm, scaling, FP32 working matrix
   hipLaunchKernelGGL(symmetrize_kernel, dim3(blocks(n, 32), blocks(n, 32)), dim3(32, 8), 0, s, layout, uplo, n,
