@@ -1215,29 +1215,87 @@ __global__ void unpack_add_masked_kernel(int64_t n, int nr, const float* __restr
   if ((mask >> q) & 1u) D[q * ldd + i] += d[i] * (double)Y[e];
 }
 
-// C (k0 x nr, ld nr) -= A^T B, A (kb x k0, row-major, ld lda), B (kb x nr, ld nr), kb <= NB_MIX, nr <= 16:
-// one thread per row of C, B staged in LDS
+// C (k0 x nr, ld nr) -= A^T B, A (kb x k0, row-major, ld lda), B (kb x nr, ld nr), kb <= NB_MIX, nr <= 16.
+// A workgroup owns 32 columns of A (32 rows of C); its 8 thread rows split the
+// kb rows of A (lane c of a thread row reads column c, so every row read is
+// 128 contiguous bytes), and the 8 partial sums meet in LDS.
 constexpr int MAXR = 16;
 __global__ void __launch_bounds__(256) atb_sub_kernel(int64_t k0, int kb, int nr, const float* __restrict__ A,
                                                       int64_t lda, const float* __restrict__ B,
                                                       float* __restrict__ C) {
   __shared__ float Bs[NB_MIX * MAXR];   // kb <= NB_MIX (checked by the caller)
+  __shared__ float part[8][32][MAXR + 1];
   for (int e = threadIdx.x; e < kb * nr; e += blockDim.x) Bs[e] = B[e];
   __syncthreads();
-  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (c >= k0) return;
+  const int cc = threadIdx.x & 31, rs = threadIdx.x >> 5;
+  const int64_t c = blockIdx.x * (int64_t)32 + cc;
   float acc[MAXR];
 #pragma unroll
   for (int q = 0; q < MAXR; ++q) acc[q] = 0.0f;
-  for (int r = 0; r < kb; ++r) {
-    const float av = A[r * lda + c];
+  if (c < k0) {
+    for (int r = rs; r < kb; r += 8) {
+      const float av = A[r * lda + c];
 #pragma unroll
-    for (int q = 0; q < MAXR; ++q)
-      if (q < nr) acc[q] = fmaf(av, Bs[r * nr + q], acc[q]);
+      for (int q = 0; q < MAXR; ++q)
+        if (q < nr) acc[q] = fmaf(av, Bs[r * nr + q], acc[q]);
+    }
   }
 #pragma unroll
-  for (int q = 0; q < MAXR; ++q)
-    if (q < nr) C[c * nr + q] -= acc[q];
+  for (int q = 0; q < MAXR; ++q) part[rs][cc][q] = acc[q];
+  __syncthreads();
+  // thread (row rs, lane cc) finishes entries q = rs, rs + 8 of column cc
+  for (int q = rs; q < nr; q += 8) {
+    float v = 0.0f;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) v += part[g][cc][q];
+    if (c < k0) C[c * nr + q] -= v;
+  }
+}
+
+// Y (n x nr, column-major, ld ldy) = alpha S X + beta Y for a row-major S (ld lds)
+// and nr <= 16 columns of X (column-major, ld ldx), S read once: a workgroup
+// owns 64 rows and walks the columns in 64-wide tiles staged through LDS
+// (coalesced row reads); thread (row r, group g) accumulates right-hand sides
+// 4g .. 4g + 3 of row r.
+__global__ void __launch_bounds__(256) skinny_gemm_f64_kernel(int64_t n, int nr, double alpha,
+                                                              const double* __restrict__ S, int64_t lds,
+                                                              const double* __restrict__ X, int64_t ldx,
+                                                              double beta, double* __restrict__ Y,
+                                                              int64_t ldy) {
+  __shared__ double st[64][65];
+  __shared__ double xs[64][MAXR];
+  const int t = threadIdx.x, r = t >> 2, qg = (t & 3) * 4;
+  const int64_t r0 = blockIdx.x * (int64_t)64;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t j0 = 0; j0 < n; j0 += 64) {
+    const int jc = (int)std::min<int64_t>(64, n - j0);
+    for (int e = t; e < 64 * 64; e += 256) {
+      const int rr = e >> 6, cc = e & 63;
+      st[rr][cc] = (r0 + rr < n && cc < jc) ? S[(r0 + rr) * lds + j0 + cc] : 0.0;
+    }
+    for (int e = t; e < 64 * MAXR; e += 256) {
+      const int j = e & 63, q = e >> 6;
+      xs[j][q] = (j < jc && q < nr) ? X[q * ldx + j0 + j] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int j = 0; j < 64; ++j) {
+      const double sv = st[r][j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = fma(sv, xs[j][qg + i], acc[i]);
+    }
+    __syncthreads();
+  }
+  if (r0 + r < n) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int q = qg + i;
+      if (q < nr) {
+        double* y = Y + q * ldy + r0 + r;
+        *y = beta == 0.0 ? alpha * acc[i] : beta * *y + alpha * acc[i];
+      }
+    }
+  }
 }
 
 // out[q] = max_i |M[q ld + i]| (out zeroed by the caller)
@@ -1381,7 +1439,7 @@ void chol_trsm_block(const Solver& S, int nr, float* Y) {
     ZH_CHECK(hipMemcpyAsync(Y + k0 * nr, S.tbuf, sizeof(float) * kb * nr, hipMemcpyDeviceToDevice, s));
     if (k0 > 0) {
       if (kb > NB_MIX || nr > MAXR) throw std::logic_error("chol_trsm_block: block too large");
-      hipLaunchKernelGGL(atb_sub_kernel, dim3(blocks(k0)), dim3(256), 0, s, k0, (int)kb, nr,
+      hipLaunchKernelGGL(atb_sub_kernel, dim3(blocks(k0, 32)), dim3(256), 0, s, k0, (int)kb, nr,
                          (const float*)(S.W + k0 * n), n, (const float*)(Y + k0 * nr), Y);
       ZH_LAUNCH_CHECK();
     }
@@ -1768,7 +1826,9 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
     for (;; ++it) {
       // R = B - S X (FP64), per-column backward errors
       ZH_CHECK(hipMemcpyAsync(Rc, Bc, sizeof(double) * n * nr, hipMemcpyDeviceToDevice, s));
-      gemm_f64_cm(kNoTrans, kNoTrans, n, nr, n, -1.0, S64, n, Xc, n, 1.0, Rc, n, s);
+      hipLaunchKernelGGL(skinny_gemm_f64_kernel, dim3(blocks(n, 64)), dim3(256), 0, s, n, nr, -1.0,
+                         (const double*)S64, n, (const double*)Xc, n, 1.0, Rc, n);
+      ZH_LAUNCH_CHECK();
       ZH_CHECK(hipMemsetAsync(cmax, 0, sizeof(double) * 2 * MAXR, s));
       hipLaunchKernelGGL(colmax_kernel, dim3(64, nr), dim3(256), 0, s, n, (const double*)Rc, n, cmax);
       hipLaunchKernelGGL(colmax_kernel, dim3(64, nr), dim3(256), 0, s, n, (const double*)Xc, n, cmax + MAXR);
