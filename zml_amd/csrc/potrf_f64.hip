@@ -53,11 +53,14 @@ potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* _
     const double r = 1.0 / S[j * ls + j];
     for (int i = j + 1 + t; i < nb; i += blockDim.x) S[i * ls + j] = S[i * ls + j] * r;
     __syncthreads();
-    // trailing lower triangle: S[i][c] -= S[i][j] * S[c][j], j < c <= i
-    const int rem = nb - j - 1;
-    for (int e = t; e < rem * rem; e += blockDim.x) {
-      const int i = j + 1 + e % rem, c = j + 1 + e / rem;
-      if (c <= i) S[i * ls + c] = S[i * ls + c] - S[i * ls + j] * S[c * ls + j];
+    // trailing lower triangle: S[i][c] -= S[i][j] * S[c][j], j < c <= i; a 16 x 16
+    // thread grid walks it (rows by ty, columns by tx) without index division
+    {
+      const int tx = t & 15, ty = t >> 4;
+      for (int i = j + 1 + ty; i < nb; i += 16) {
+        const double lij = S[i * ls + j];
+        for (int c = j + 1 + tx; c <= i; c += 16) S[i * ls + c] = S[i * ls + c] - lij * S[c * ls + j];
+      }
     }
     __syncthreads();
   }
