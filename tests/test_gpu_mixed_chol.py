@@ -110,6 +110,23 @@ def test_mixed_cholesky_layouts(zh, torch_cuda, order, uplo):
     assert np.abs(X - Xo).max() / np.abs(Xo).max() < 1e-9
 
 
+@pytest.mark.parametrize("n", [333, 130, 1])
+def test_mixed_cholesky_unaligned_orders(zh, torch_cuda, n):
+    """Orders that are not multiples of 4 (scalar load paths of the solves and
+    the residual), a single partial block and the 1 x 1 system."""
+    nrhs = 7
+    A = spd_matrix(n, 1e-2, 21)
+    B = np.random.default_rng(48).uniform(-1, 1, (n, nrhs))
+    X, res = run_posv_mixed(zh, torch_cuda, A, B, ROW, b"L", 0)
+    assert backward_errors(A, X, B).max() <= 16
+    if n > 1:
+        # (for n = 1 the rounding of x alone leaves a backward error near the default
+        # target of 1, so refinement may hand over to GMRES and FP64)
+        assert res.iterations <= 3 and res.gmres_iterations == 0
+    Xo = oracle_posv(A, B)
+    assert np.abs(X - Xo).max() / np.abs(Xo).max() < 1e-10
+
+
 def test_mixed_cholesky_many_rhs_groups(zh, torch_cuda):
     """More than one group of 16 right-hand sides."""
     n, nrhs = 520, 37

@@ -955,7 +955,7 @@ struct Solver {
   // Cholesky only
   float* invLT = nullptr;      // transposed diagonal-block inverses of L
   float* ybuf = nullptr;       // (n + NB) x 16 right-hand sides, FP32 row-major
-  float* tbuf = nullptr;       // NB x 16
+  double* pbuf = nullptr;      // split partial products of the FP64 residual (16 x n x splits)
 };
 
 double reduce_host(const Solver& S, int64_t n, const double* x, const double* y, int mode) {
@@ -1215,87 +1215,284 @@ __global__ void unpack_add_masked_kernel(int64_t n, int nr, const float* __restr
   if ((mask >> q) & 1u) D[q * ldd + i] += d[i] * (double)Y[e];
 }
 
-// C (k0 x nr, ld nr) -= A^T B, A (kb x k0, row-major, ld lda), B (kb x nr, ld nr), kb <= NB_MIX, nr <= 16.
-// A workgroup owns 32 columns of A (32 rows of C); its 8 thread rows split the
-// kb rows of A (lane c of a thread row reads column c, so every row read is
-// 128 contiguous bytes), and the 8 partial sums meet in LDS.
-constexpr int MAXR = 16;
-__global__ void __launch_bounds__(256) atb_sub_kernel(int64_t k0, int kb, int nr, const float* __restrict__ A,
-                                                      int64_t lda, const float* __restrict__ B,
-                                                      float* __restrict__ C) {
-  __shared__ float Bs[NB_MIX * MAXR];   // kb <= NB_MIX (checked by the caller)
-  __shared__ float part[8][32][MAXR + 1];
-  for (int e = threadIdx.x; e < kb * nr; e += blockDim.x) Bs[e] = B[e];
-  __syncthreads();
-  const int cc = threadIdx.x & 31, rs = threadIdx.x >> 5;
-  const int64_t c = blockIdx.x * (int64_t)32 + cc;
-  float acc[MAXR];
+constexpr int MAXR = 16;   // right-hand sides per group of the Cholesky solve
+
+// ---- triangular solves with up to 16 right-hand sides (persistent, FP32 MFMA) -------------
+// L z = y (trans 0) or L^T z = y (trans 1) for the row-major lower factor L in W
+// and Y (n x nr, FP32 row-major, ld ldy) overwritten by z. Block rows of T16
+// are dealt to the workgroups in dependency order (as in trisolve_f32_kernel);
+// wave w owns rows 32w .. 32w + 31 of its block and all 16 right-hand sides as
+// two 16 x 16 accumulator tiles of v_mfma_f32_16x16x4f32. A finished block is
+// published as n x 16 tagged words (tag << 32 | float), which a consumer
+// workgroup stages once per column block in LDS. z_b = D_b (y_b - sum), D_b the
+// T16 x T16 diagonal sub-block of the stored inverse of the enclosing NB_MIX
+// block (inverse of L_BB for trans 0, its transpose for trans 1).
+// Operand mapping of the MFMA (A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane j + 16 (i / 4), register i % 4): for trans 0 lane (i, kk)
+// walks 4 consecutive columns c = 16 m + 4 kk + e of its row (one 16-byte load
+// when the row start is aligned); for trans 1 it reads L[c][r0 + i] with
+// c = 4 m + kk, 16 lanes covering 64 contiguous bytes of a row of L.
+constexpr int T16 = 128;
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+struct TriSolve16 {
+  int64_t n, nblk;
+  int trans, nr;
+  const float* W;
+  int64_t ldw;
+  const float* inv;
+  float* Y;
+  int64_t ldy;
+  uint64_t* xch;
+  uint32_t tag;
+  int* err;
+  int64_t spin_limit;
+};
+
+__device__ __forceinline__ float wait_word16(const uint64_t* p, uint32_t tag, int64_t limit, int* err) {
+  return wait_word(p, tag, limit, err);
+}
+
+template <bool VEC>
+__device__ __forceinline__ f4v load_row4(const float* p, int valid) {
+  if (VEC && valid >= 4) return *reinterpret_cast<const f4v*>(p);
+  f4v v = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int q = 0; q < MAXR; ++q) acc[q] = 0.0f;
-  if (c < k0) {
-    for (int r = rs; r < kb; r += 8) {
-      const float av = A[r * lda + c];
+  for (int e = 0; e < 4; ++e)
+    if (e < valid) v[e] = p[e];
+  return v;
+}
+
+template <bool VEC, bool TRANS>
+__global__ void __launch_bounds__(256) trisolve16_kernel(TriSolve16 a) {
+  __shared__ float zs[2][T16][17];
+  __shared__ float ts[T16][17];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int li = lane & 15, kk = lane >> 4;
+  const int wr = w * 32;
+  for (int64_t bi = blockIdx.x; bi < a.nblk; bi += gridDim.x) {
+    const int64_t b = TRANS ? a.nblk - 1 - bi : bi;
+    const int64_t r0 = b * T16;
+    const int rows = (int)std::min<int64_t>(T16, a.n - r0);
+    // D_b rows wr + 16u + li, columns 16 m + 4 kk + e
+    const int64_t B = r0 / NB_MIX;
+    const int64_t o = r0 - B * NB_MIX;
+    const int64_t ldi = std::min<int64_t>(NB_MIX, a.n - B * NB_MIX);
+    const float* D = a.inv + B * NB_MIX * NB_MIX + o * ldi + o;
+    const bool dvec = VEC && (ldi & 3) == 0;
+    f4v dreg[2][8];
 #pragma unroll
-      for (int q = 0; q < MAXR; ++q)
-        if (q < nr) acc[q] = fmaf(av, Bs[r * nr + q], acc[q]);
+    for (int u = 0; u < 2; ++u) {
+      const int r = wr + 16 * u + li;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int c = 16 * m + 4 * kk;
+        const int valid = r < rows ? rows - c : 0;
+        dreg[u][m] = dvec ? load_row4<true>(D + r * ldi + c, valid) : load_row4<false>(D + r * ldi + c, valid);
+      }
     }
-  }
+    f4v acc[2];
+    acc[0] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    acc[1] = acc[0];
+    const int64_t cnt = TRANS ? a.nblk - 1 - b : b;
+    for (int64_t q = 0; q < cnt; ++q) {
+      const int64_t j = TRANS ? a.nblk - 1 - q : q;
+      const int64_t c0 = j * T16;
+      const int cols = (int)std::min<int64_t>(T16, a.n - c0);
+      const int buf = (int)(q & 1);
+      // the factor's values first (independent of z_j)
+      float lv[2][32];
+      if (!TRANS) {
 #pragma unroll
-  for (int q = 0; q < MAXR; ++q) part[rs][cc][q] = acc[q];
-  __syncthreads();
-  // thread (row rs, lane cc) finishes entries q = rs, rs + 8 of column cc
-  for (int q = rs; q < nr; q += 8) {
-    float v = 0.0f;
+        for (int u = 0; u < 2; ++u) {
+          const int r = wr + 16 * u + li;
+          const float* row = a.W + (r0 + r) * a.ldw + c0;
 #pragma unroll
-    for (int g = 0; g < 8; ++g) v += part[g][cc][q];
-    if (c < k0) C[c * nr + q] -= v;
+          for (int m = 0; m < 8; ++m) {
+            const int c = 16 * m + 4 * kk;
+            const f4v v = load_row4<VEC>(row + c, r < rows ? cols - c : 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) lv[u][4 * m + e] = v[e];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int r = wr + 16 * u + li;
+#pragma unroll
+          for (int m = 0; m < 32; ++m) {
+            const int c = 4 * m + kk;
+            lv[u][m] = (r < rows && c < cols) ? a.W[(c0 + c) * a.ldw + r0 + r] : 0.0f;
+          }
+        }
+      }
+      // z_j into LDS: 8 words per thread, consecutive threads on consecutive words
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = t + 256 * u;
+        const int c = e >> 4, qq = e & 15;
+        zs[buf][c][qq] = c < cols ? wait_word16(a.xch + (c0 + c) * 16 + qq, a.tag, a.spin_limit, a.err) : 0.0f;
+      }
+      __syncthreads();
+      if (!TRANS) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(lv[u][4 * m + e], zs[buf][16 * m + 4 * kk + e][li],
+                                                            acc[u], 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int m = 0; m < 32; ++m)
+            acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(lv[u][m], zs[buf][4 * m + kk][li], acc[u], 0, 0, 0);
+      }
+    }
+    // t = y - sum (rows 16u + 4 kk + reg of this wave, right-hand side li)
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int r = wr + 16 * u + 4 * kk + g;
+        const float y = (r < rows && li < a.nr) ? a.Y[(r0 + r) * a.ldy + li] : 0.0f;
+        ts[r][li] = r < rows ? y - acc[u][g] : 0.0f;
+      }
+    __syncthreads();
+    // z_b = D_b t
+    f4v z[2];
+    z[0] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    z[1] = z[0];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          z[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(dreg[u][m][e], ts[16 * m + 4 * kk + e][li], z[u], 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int r = wr + 16 * u + 4 * kk + g;
+        if (r < rows) {
+          const int64_t i = r0 + r;
+          __hip_atomic_store(a.xch + i * 16 + li, ((uint64_t)a.tag << 32) | __float_as_uint(z[u][g]),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (li < a.nr) a.Y[i * a.ldy + li] = z[u][g];
+        }
+      }
+    __syncthreads();
   }
 }
 
-// Y (n x nr, column-major, ld ldy) = alpha S X + beta Y for a row-major S (ld lds)
-// and nr <= 16 columns of X (column-major, ld ldx), S read once: a workgroup
-// owns 64 rows and walks the columns in 64-wide tiles staged through LDS
-// (coalesced row reads); thread (row r, group g) accumulates right-hand sides
-// 4g .. 4g + 3 of row r.
-__global__ void __launch_bounds__(256) skinny_gemm_f64_kernel(int64_t n, int nr, double alpha,
-                                                              const double* __restrict__ S, int64_t lds,
-                                                              const double* __restrict__ X, int64_t ldx,
-                                                              double beta, double* __restrict__ Y,
-                                                              int64_t ldy) {
-  __shared__ double st[64][65];
-  __shared__ double xs[64][MAXR];
-  const int t = threadIdx.x, r = t >> 2, qg = (t & 3) * 4;
-  const int64_t r0 = blockIdx.x * (int64_t)64;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t j0 = 0; j0 < n; j0 += 64) {
-    const int jc = (int)std::min<int64_t>(64, n - j0);
-    for (int e = t; e < 64 * 64; e += 256) {
-      const int rr = e >> 6, cc = e & 63;
-      st[rr][cc] = (r0 + rr < n && cc < jc) ? S[(r0 + rr) * lds + j0 + cc] : 0.0;
-    }
-    for (int e = t; e < 64 * MAXR; e += 256) {
-      const int j = e & 63, q = e >> 6;
-      xs[j][q] = (j < jc && q < nr) ? X[q * ldx + j0 + j] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int j = 0; j < 64; ++j) {
-      const double sv = st[r][j];
+// ---- FP64 residual products with up to 16 right-hand sides ---------------------------------
+// P[s] (16 x n, ld n) = S[:, ks] X[ks, :] over the column range ks of split s,
+// S row-major (ld lds), X column-major (ld ldx, nr <= 16 columns). A workgroup
+// owns 64 rows (wave w: rows 16 w .. 16 w + 15) of one split and walks it in
+// 64-column steps: the X slice is staged in LDS, S is read straight into the
+// v_mfma_f64_16x16x4f64 A operand, lane (i, kk) taking columns 8 m + 2 kk + h
+// (m = 0..7, h = 0..1) of its row as 16-byte loads, the next step's loads
+// issued before the current step's products.
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+template <bool VEC>
+__global__ void __launch_bounds__(256) skinny_mfma_f64_kernel(int64_t n, int nr, const double* __restrict__ S,
+                                                              int64_t lds, const double* __restrict__ X,
+                                                              int64_t ldx, int64_t kper, double* __restrict__ P) {
+  __shared__ double xs[2][64][17];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, kk = lane >> 4;
+  const int64_t row = blockIdx.x * (int64_t)64 + w * 16 + li;
+  const int64_t kbeg = blockIdx.y * kper, kend = std::min<int64_t>(n, kbeg + kper);
+  const double* srow = S + (row < n ? row : 0) * lds;
+  auto load_s = [&](int64_t k0, d2v* sv) {
+    const bool full = row < n && k0 + 64 <= kend;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i] = fma(sv, xs[j][qg + i], acc[i]);
-    }
-    __syncthreads();
-  }
-  if (r0 + r < n) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int q = qg + i;
-      if (q < nr) {
-        double* y = Y + q * ldy + r0 + r;
-        *y = beta == 0.0 ? alpha * acc[i] : beta * *y + alpha * acc[i];
+    for (int m = 0; m < 8; ++m) {
+      const int64_t c = k0 + 8 * m + 2 * kk;
+      if (VEC && full) {
+        sv[m] = *reinterpret_cast<const d2v*>(srow + c);
+      } else {
+        sv[m][0] = (row < n && c < kend) ? srow[c] : 0.0;
+        sv[m][1] = (row < n && c + 1 < kend) ? srow[c + 1] : 0.0;
       }
     }
+  };
+  d4v acc = {0.0, 0.0, 0.0, 0.0};
+  d2v cur[8], nxt[8];
+  if (kbeg < kend) load_s(kbeg, nxt);
+  int it = 0;
+  for (int64_t k0 = kbeg; k0 < kend; k0 += 64, ++it) {
+    const int buf = it & 1;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) cur[m] = nxt[m];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u;
+      const int c = e & 63, q = e >> 6;
+      xs[buf][c][q] = (k0 + c < kend && q < nr) ? X[q * ldx + k0 + c] : 0.0;
+    }
+    if (k0 + 64 < kend) load_s(k0 + 64, nxt);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[m][h], xs[buf][8 * m + 2 * kk + h][li], acc, 0, 0, 0);
   }
+  // D[i][j] in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int64_t r = blockIdx.x * (int64_t)64 + w * 16 + 4 * g + kk;
+    if (r < n && li < nr) P[(blockIdx.y * (int64_t)16 + li) * n + r] = acc[g];
+  }
+}
+
+// Y[:, q] = beta Y[:, q] + alpha sum_s P[s][q] (splits summed in order)
+__global__ void skinny_reduce_kernel(int64_t n, int nr, int splits, const double* __restrict__ P, double alpha,
+                                     double beta, double* __restrict__ Y, int64_t ldy) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n * nr) return;
+  const int64_t r = e % n;
+  const int q = (int)(e / n);
+  double acc = 0.0;
+  for (int sp = 0; sp < splits; ++sp) acc += P[(sp * (int64_t)16 + q) * n + r];
+  double* y = Y + q * ldy + r;
+  *y = beta == 0.0 ? alpha * acc : beta * *y + alpha * acc;
+}
+
+// column split of the residual product: enough workgroups to cover the CUs a few times
+struct SkinnySplit {
+  int64_t kper;
+  int splits;
+};
+SkinnySplit skinny_split(int64_t n) {
+  const int64_t rb = (n + 63) / 64;
+  int64_t want = std::max<int64_t>(1, (2048 + rb - 1) / rb);
+  want = std::min<int64_t>(want, (n + 63) / 64);
+  int64_t kper = ((n + want - 1) / want + 63) / 64 * 64;
+  return SkinnySplit{kper, (int)((n + kper - 1) / kper)};
+}
+
+// Y (n x nr, column-major, ld ldy) = beta Y + alpha S X with P as scratch
+// (16 x n x splits doubles)
+void skinny_gemm_f64(const Solver& S, int nr, double alpha, const double* A, int64_t lda, const double* X,
+                     int64_t ldx, double beta, double* Y, int64_t ldy, double* P) {
+  const int64_t n = S.n;
+  const SkinnySplit sp = skinny_split(n);
+  const dim3 grid((unsigned)((n + 63) / 64), (unsigned)sp.splits);
+  const bool vec = (lda & 1) == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+  if (vec)
+    hipLaunchKernelGGL(skinny_mfma_f64_kernel<true>, grid, dim3(256), 0, S.s, n, nr, A, lda, X, ldx, sp.kper, P);
+  else
+    hipLaunchKernelGGL(skinny_mfma_f64_kernel<false>, grid, dim3(256), 0, S.s, n, nr, A, lda, X, ldx, sp.kper, P);
+  ZH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(skinny_reduce_kernel, dim3(blocks(n * nr)), dim3(256), 0, S.s, n, nr, sp.splits,
+                     (const double*)P, alpha, beta, Y, ldy);
+  ZH_LAUNCH_CHECK();
 }
 
 // out[q] = max_i |M[q ld + i]| (out zeroed by the caller)
@@ -1419,30 +1616,33 @@ void chol_factor(Solver& S, const CholWork& cw, int prec, double shift) {
   ZH_CHECK(hipStreamWaitEvent(caller, ev_out, 0));
 }
 
-// Y (n x nr, FP32 row-major, ld nr) <- L^-T L^-1 Y with the blocked factor in S.W
+// Y (n x nr, FP32 row-major, ld nr) <- L^-T L^-1 Y with the factor in S.W: two
+// persistent launches of trisolve16_kernel
 void chol_trsm_block(const Solver& S, int nr, float* Y) {
-  const int64_t n = S.n;
-  hipStream_t s = S.s;
-  for (int64_t k0 = 0, b = 0; k0 < n; k0 += NB_MIX, ++b) {   // L z = y
-    const int64_t kb = std::min<int64_t>(NB_MIX, n - k0);
-    mp_gemm_f32(kb, nr, kb, 1.0f, S.invL + b * NB_MIX * NB_MIX, kb, Y + k0 * nr, nr, 0.0f, nullptr, nullptr,
-                S.tbuf, nr, s);
-    ZH_CHECK(hipMemcpyAsync(Y + k0 * nr, S.tbuf, sizeof(float) * kb * nr, hipMemcpyDeviceToDevice, s));
-    if (k0 + kb < n)
-      mp_gemm_f32(n - k0 - kb, nr, kb, -1.0f, S.W + (k0 + kb) * n + k0, n, Y + k0 * nr, nr, 1.0f, nullptr,
-                  nullptr, Y + (k0 + kb) * nr, nr, s);
-  }
-  for (int64_t b = S.nblk - 1; b >= 0; --b) {                  // L^T x = z
-    const int64_t k0 = b * NB_MIX, kb = std::min<int64_t>(NB_MIX, n - k0);
-    mp_gemm_f32(kb, nr, kb, 1.0f, S.invLT + b * NB_MIX * NB_MIX, kb, Y + k0 * nr, nr, 0.0f, nullptr, nullptr,
-                S.tbuf, nr, s);
-    ZH_CHECK(hipMemcpyAsync(Y + k0 * nr, S.tbuf, sizeof(float) * kb * nr, hipMemcpyDeviceToDevice, s));
-    if (k0 > 0) {
-      if (kb > NB_MIX || nr > MAXR) throw std::logic_error("chol_trsm_block: block too large");
-      hipLaunchKernelGGL(atb_sub_kernel, dim3(blocks(k0, 32)), dim3(256), 0, s, k0, (int)kb, nr,
-                         (const float*)(S.W + k0 * n), n, (const float*)(Y + k0 * nr), Y);
-      ZH_LAUNCH_CHECK();
-    }
+  if (nr < 1 || nr > MAXR) throw std::logic_error("chol_trsm_block: 1..16 right-hand sides");
+  TriSolve16 f{};
+  f.n = S.n;
+  f.nblk = (S.n + T16 - 1) / T16;
+  f.trans = 0;
+  f.nr = nr;
+  f.W = S.W;
+  f.ldw = S.n;
+  f.inv = S.invL;
+  f.Y = Y;
+  f.ldy = nr;
+  f.xch = S.xch;
+  f.err = S.serr;
+  f.spin_limit = int64_t(1) << 22;
+  const bool vec = (S.n & 3) == 0;
+  for (int dir = 0; dir < 2; ++dir) {
+    f.trans = dir;
+    f.inv = dir ? S.invLT : S.invL;
+    f.tag = ++S.tag;
+    if (vec && dir) hipLaunchKernelGGL((trisolve16_kernel<true, true>), dim3(S.ts_grid), dim3(256), 0, S.s, f);
+    else if (vec) hipLaunchKernelGGL((trisolve16_kernel<true, false>), dim3(S.ts_grid), dim3(256), 0, S.s, f);
+    else if (dir) hipLaunchKernelGGL((trisolve16_kernel<false, true>), dim3(S.ts_grid), dim3(256), 0, S.s, f);
+    else hipLaunchKernelGGL((trisolve16_kernel<false, false>), dim3(S.ts_grid), dim3(256), 0, S.s, f);
+    ZH_LAUNCH_CHECK();
   }
 }
 
@@ -1741,10 +1941,12 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
   S.lda = n;
   S.W = static_cast<float*>(workspace(sizeof(float) * (size_t)n * n, 8));
   const size_t blk2 = (size_t)NB_MIX * NB_MIX;
+  const SkinnySplit ssp = skinny_split(n);
   char* small = static_cast<char*>(workspace(
       sizeof(float) * 2 * S.nblk * blk2 + sizeof(double) * 2 * n + sizeof(double) * 512 +
-          sizeof(float) * ((n + NB_MIX) * MAXR + NB_MIX * MAXR) + sizeof(double) * 2 * blk2 +
-          sizeof(int) * (S.nblk + 64) + sizeof(double) * 2 * MAXR + 4096, 9));
+          sizeof(float) * (n + NB_MIX) * MAXR + sizeof(uint64_t) * (n + NB_MIX) * MAXR +
+          sizeof(double) * MAXR * n * ssp.splits + sizeof(double) * 2 * blk2 + sizeof(int) * (S.nblk + 64) +
+          sizeof(double) * 2 * MAXR + 16 * 256, 9));
   auto carve = [&](size_t bytes) { char* p = small; small += (bytes + 255) / 256 * 256; return p; };
   S.invL = reinterpret_cast<float*>(carve(sizeof(float) * S.nblk * blk2));
   S.invLT = reinterpret_cast<float*>(carve(sizeof(float) * S.nblk * blk2));
@@ -1753,7 +1955,26 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
   S.rowsum = reinterpret_cast<double*>(carve(sizeof(double) * n));
   S.part = reinterpret_cast<double*>(carve(sizeof(double) * 512));
   S.ybuf = reinterpret_cast<float*>(carve(sizeof(float) * (n + NB_MIX) * MAXR));
-  S.tbuf = reinterpret_cast<float*>(carve(sizeof(float) * NB_MIX * MAXR));
+  S.xch = reinterpret_cast<uint64_t*>(carve(sizeof(uint64_t) * (n + NB_MIX) * MAXR));
+  S.serr = reinterpret_cast<int*>(carve(64));
+  S.pbuf = reinterpret_cast<double*>(carve(sizeof(double) * MAXR * n * ssp.splits));
+  // tags start again from 1 on cleared exchange words
+  ZH_CHECK(hipMemsetAsync(S.xch, 0, sizeof(uint64_t) * (n + NB_MIX) * MAXR, s));
+  ZH_CHECK(hipMemsetAsync(S.serr, 0, 64, s));
+  {
+    int dev = 0, cus = 0, per_cu = 0;
+    ZH_CHECK(hipGetDevice(&dev));
+    ZH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    per_cu = 1 << 20;
+    for (const void* k : {(const void*)trisolve16_kernel<true, false>, (const void*)trisolve16_kernel<true, true>,
+                          (const void*)trisolve16_kernel<false, false>, (const void*)trisolve16_kernel<false, true>}) {
+      int pc = 0;
+      ZH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k, 256, 0));
+      per_cu = std::min(per_cu, pc);
+    }
+    const int64_t nts = (n + T16 - 1) / T16;
+    S.ts_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nts, (int64_t)cus * std::max(per_cu, 1)));
+  }
   CholWork cw;
   cw.d64 = reinterpret_cast<double*>(carve(sizeof(double) * blk2));
   cw.inv64 = reinterpret_cast<double*>(carve(sizeof(double) * blk2));
@@ -1826,15 +2047,14 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
     for (;; ++it) {
       // R = B - S X (FP64), per-column backward errors
       ZH_CHECK(hipMemcpyAsync(Rc, Bc, sizeof(double) * n * nr, hipMemcpyDeviceToDevice, s));
-      hipLaunchKernelGGL(skinny_gemm_f64_kernel, dim3(blocks(n, 64)), dim3(256), 0, s, n, nr, -1.0,
-                         (const double*)S64, n, (const double*)Xc, n, 1.0, Rc, n);
-      ZH_LAUNCH_CHECK();
+      skinny_gemm_f64(S, nr, -1.0, S64, n, Xc, n, 1.0, Rc, n, S.pbuf);
       ZH_CHECK(hipMemsetAsync(cmax, 0, sizeof(double) * 2 * MAXR, s));
       hipLaunchKernelGGL(colmax_kernel, dim3(64, nr), dim3(256), 0, s, n, (const double*)Rc, n, cmax);
       hipLaunchKernelGGL(colmax_kernel, dim3(64, nr), dim3(256), 0, s, n, (const double*)Xc, n, cmax + MAXR);
       ZH_LAUNCH_CHECK();
       ZH_CHECK(hipMemcpyAsync(hm.data(), cmax, sizeof(double) * 2 * MAXR, hipMemcpyDeviceToHost, s));
       ZH_CHECK(hipStreamSynchronize(s));
+      check_solve(S);
       bool active = false;
       for (int q = 0; q < nr; ++q) {
         if (state[q] != 0) continue;

@@ -24,55 +24,95 @@ namespace {
 
 constexpr int DB = 128;   // diagonal block of the driver
 
-// Unblocked Cholesky of an nb x nb block (nb <= DB) held in LDS.
+// Unblocked Cholesky of an nb x nb block (nb <= DB), register resident.
 // lower: A = L L^T on the lower triangle; else A = U^T U on the upper one.
+// Thread (tx, ty) of a 16 x 16 grid holds the elements (tx + 16 ii, ty + 16 cc),
+// ii >= cc, of the lower-orientation block in registers. Column j travels
+// through LDS unscaled (double buffered, one barrier per column); every thread
+// takes l_jj = sqrt(a_jj), r = fl(1 / l_jj) and l_ij = fl(a_ij * r) from it, so
+// the trailing update a_ic - l_ij * l_cj uses the same separately rounded
+// values, in the same right-looking order, as the unblocked algorithm.
 __global__ void __launch_bounds__(256)
 potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* __restrict__ info,
                   int64_t info_base) {
 #pragma clang fp contract(off)
-  __shared__ double S[DB * (DB + 1)];
-  __shared__ int s_fail;
+  __shared__ double colbuf[2][DB];
   if (*info != 0) return;   // an earlier block already failed
-  const int t = threadIdx.x;
-  const int ls = DB + 1;
-  // S holds the block in "lower" orientation: S[r*ls + c] = L-view element (r, c), r >= c
-  for (int e = t; e < nb * nb; e += blockDim.x) {
-    const int r = e % nb, c = e / nb;
-    if (r >= c) S[r * ls + c] = lower ? a[r + (int64_t)c * lda] : a[c + (int64_t)r * lda];
-  }
-  if (t == 0) s_fail = 0;
-  __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    if (t == 0) {
-      const double ajj = S[j * ls + j];
-      if (ajj <= 0.0 || ajj != ajj) s_fail = j + 1;
-      else S[j * ls + j] = sqrt(ajj);
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  double v[8][8];
+#pragma unroll
+  for (int ii = 0; ii < 8; ++ii)
+#pragma unroll
+    for (int cc = 0; cc <= ii; ++cc) {
+      const int r = tx + 16 * ii, c = ty + 16 * cc;
+      v[ii][cc] = (r < nb && c <= r) ? (lower ? a[r + (int64_t)c * lda] : a[c + (int64_t)r * lda]) : 0.0;
     }
-    __syncthreads();
-    if (s_fail) break;
-    const double r = 1.0 / S[j * ls + j];
-    for (int i = j + 1 + t; i < nb; i += blockDim.x) S[i * ls + j] = S[i * ls + j] * r;
-    __syncthreads();
-    // trailing lower triangle: S[i][c] -= S[i][j] * S[c][j], j < c <= i; a 16 x 16
-    // thread grid walks it (rows by ty, columns by tx) without index division
-    {
-      const int tx = t & 15, ty = t >> 4;
-      for (int i = j + 1 + ty; i < nb; i += 16) {
-        const double lij = S[i * ls + j];
-        for (int c = j + 1 + tx; c <= i; c += 16) S[i * ls + c] = S[i * ls + c] - lij * S[c * ls + j];
+  // column 0 (ty == 0, cc == 0)
+  if (ty == 0) {
+#pragma unroll
+    for (int ii = 0; ii < 8; ++ii)
+      if (tx + 16 * ii < nb) colbuf[0][tx + 16 * ii] = v[ii][0];
+  }
+  __syncthreads();
+  int fail = 0;
+  for (int j = 0; j < nb; ++j) {
+    const double* cur = colbuf[j & 1];
+    const double d = cur[j];
+    if (!(d > 0.0)) { fail = j + 1; break; }   // non-positive or NaN pivot
+    const double ljj = sqrt(d);
+    const double rj = 1.0 / ljj;
+    double lr[8], lc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      lr[k] = cur[tx + 16 * k] * rj;
+      lc[k] = cur[ty + 16 * k] * rj;
+    }
+    const int jc = j >> 4, jt = j & 15;
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) {
+      if (cc == jc && ty == jt) {   // this thread holds part of column j: finalize it
+#pragma unroll
+        for (int ii = cc; ii < 8; ++ii) {
+          const int r = tx + 16 * ii;
+          if (r == j) v[ii][cc] = ljj;
+          else if (r > j) v[ii][cc] = lr[ii];
+        }
+      }
+      if (16 * cc + 15 > j) {   // some column of this group lies right of j
+        const int c = ty + 16 * cc;
+#pragma unroll
+        for (int ii = cc; ii < 8; ++ii) {
+          const int r = tx + 16 * ii;
+          if (c > j && c <= r && r < nb) v[ii][cc] = v[ii][cc] - lr[ii] * lc[cc];
+        }
       }
     }
+    // publish column j + 1, now fully updated
+    const int jn = j + 1;
+    if (jn < nb && ty == (jn & 15)) {
+#pragma unroll
+      for (int cc = 0; cc < 8; ++cc)
+        if (cc == (jn >> 4)) {
+#pragma unroll
+          for (int ii = cc; ii < 8; ++ii) {
+            const int r = tx + 16 * ii;
+            if (r >= jn && r < nb) colbuf[jn & 1][r] = v[ii][cc];
+          }
+        }
+    }
     __syncthreads();
   }
-  const int fail = s_fail;
   const int done = fail ? fail - 1 : nb;   // columns that were completed
-  for (int e = t; e < nb * nb; e += blockDim.x) {
-    const int r = e % nb, c = e / nb;
-    if (r >= c && c < done) {
-      if (lower) a[r + (int64_t)c * lda] = S[r * ls + c];
-      else a[c + (int64_t)r * lda] = S[r * ls + c];
+#pragma unroll
+  for (int ii = 0; ii < 8; ++ii)
+#pragma unroll
+    for (int cc = 0; cc <= ii; ++cc) {
+      const int r = tx + 16 * ii, c = ty + 16 * cc;
+      if (r < nb && c <= r && c < done) {
+        if (lower) a[r + (int64_t)c * lda] = v[ii][cc];
+        else a[c + (int64_t)r * lda] = v[ii][cc];
+      }
     }
-  }
   if (fail && t == 0) *info = (int)(info_base + fail);
 }
 
