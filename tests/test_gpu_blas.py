@@ -175,7 +175,8 @@ def test_dtrsm_alpha_zero_zeroes_b(zh):
 @pytest.mark.parametrize("order", [COL, ROW])
 @pytest.mark.parametrize("uplo", UPLOS)
 @pytest.mark.parametrize("trans", [111, 112])
-@pytest.mark.parametrize("n,k,alpha,beta", [(5, 3, 2.0, 3.0), (150, 70, -1.0, 1.0), (260, 33, 0.7, 0.0)])
+@pytest.mark.parametrize("n,k,alpha,beta", [(5, 3, 2.0, 3.0), (150, 70, -1.0, 1.0), (260, 33, 0.7, 0.0),
+                                            (2100, 40, -1.0, 1.0)])
 def test_dsyrk_matches_oracle(zh, order, uplo, trans, n, k, alpha, beta):
     rng = np.random.default_rng(n + k + uplo + trans)
     Am = rng.uniform(-1, 1, (n, k)) if trans == 111 else rng.uniform(-1, 1, (k, n))

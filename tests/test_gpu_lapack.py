@@ -168,7 +168,7 @@ def _spd(n, seed, mu=1.0):
 
 @pytest.mark.parametrize("order", [ROW, COL])
 @pytest.mark.parametrize("uplo", [b"L", b"U"])
-@pytest.mark.parametrize("n", [1, 7, 128, 129, 500])
+@pytest.mark.parametrize("n", [1, 7, 128, 129, 500, 1100])
 def test_dpotrf_matches_oracle(zh, order, uplo, n):
     A = _spd(n, 40 + n)
     a = store(order, A, n)
@@ -201,10 +201,26 @@ def test_dpotrf_not_positive_definite(zh):
     assert zh.LAPACKE_dpotrf(COL, b"L", n, P(a), n) == 6
 
 
+@pytest.mark.parametrize("uplo", [b"L", b"U"])
+def test_dpotrf_not_positive_definite_later_block_column(zh, uplo):
+    """A failing pivot in the third 512-column block column."""
+    n = 1300
+    A = _spd(n, 9)
+    A[1100, 1100] = -5.0
+    a = store(COL, A, n)
+    ref = a.copy()
+    code = 122 if uplo == b"L" else 121
+    assert zh.LAPACKE_dpotrf(COL, uplo, n, P(a), n) == O.potrf(COL, code, n, ref, n) == 1101
+    got, exp = dense(COL, n, n, a, n), dense(COL, n, n, ref, n)
+    tri = np.tril if uplo == b"L" else np.triu
+    assert np.abs(tri(got)[:1100, :1100] - tri(exp)[:1100, :1100]).max() < 1e-12
+
+
 @pytest.mark.parametrize("order", [ROW, COL])
 @pytest.mark.parametrize("uplo", [b"L", b"U"])
-def test_dpotrs_dposv(zh, order, uplo):
-    n, nrhs = 400, 16
+@pytest.mark.parametrize("n", [400, 1100])
+def test_dpotrs_dposv(zh, order, uplo, n):
+    nrhs = 16
     A = _spd(n, 12)
     X = np.random.default_rng(5).uniform(-1, 1, (n, nrhs))
     B = A @ X

@@ -47,17 +47,56 @@ __device__ __forceinline__ void tile_coords(int64_t bid, int64_t ntm, int64_t nt
   tn = r / gm;
 }
 
+// Tiles of a triangular update only (tri lower with ntm >= ntn, or upper with
+// ntn >= ntm): the active tiles are enumerated compactly, in groups of 8 tile
+// columns (lower; 8 tile rows for upper) so the group's B (A) panels stay in L2,
+// and split into equal contiguous ranges per XCD like tile_coords. In group h
+// (width g), the first g rows (columns) form a triangle of 1, 2, .., g tiles,
+// the rest are full rows (columns) of g tiles.
+__host__ __device__ inline int64_t tri_group_tiles(int64_t h, int64_t nlong, int64_t nshort) {
+  const int64_t g = nshort - 8 * h < 8 ? nshort - 8 * h : 8;
+  return g * (g + 1) / 2 + (nlong - 8 * h - g) * g;
+}
+
+__device__ __forceinline__ void tri_tile_coords(int64_t bid, int64_t nt, int64_t ntm, int64_t ntn, int tri,
+                                                int64_t& tm, int64_t& tn) {
+  const int64_t x = bid & 7, q = nt >> 3, r8 = nt & 7;
+  int64_t pid = x * q + (x < r8 ? x : r8) + (bid >> 3);
+  const int64_t nlong = tri == kLower ? ntm : ntn, nshort = tri == kLower ? ntn : ntm;
+  int64_t h = 0;
+  for (;; ++h) {
+    const int64_t c = tri_group_tiles(h, nlong, nshort);
+    if (pid < c) break;
+    pid -= c;
+  }
+  const int64_t g = nshort - 8 * h < 8 ? nshort - 8 * h : 8;
+  int64_t lo, sh;   // position along the long side, along the short side
+  if (pid < g * (g + 1) / 2) {
+    int64_t k = 0;
+    while ((k + 1) * (k + 2) / 2 <= pid) ++k;
+    lo = 8 * h + k;
+    sh = 8 * h + (pid - k * (k + 1) / 2);
+  } else {
+    const int64_t r = pid - g * (g + 1) / 2;
+    lo = 8 * h + g + r / g;
+    sh = 8 * h + r % g;
+  }
+  if (tri == kLower) { tm = lo; tn = sh; }
+  else { tm = sh; tn = lo; }
+}
+
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(NT, 2)
 gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __restrict__ A,
                 int64_t lda, const double* __restrict__ B, int64_t ldb, double beta,
-                double* __restrict__ C, int64_t ldc, int tri) {
+                double* __restrict__ C, int64_t ldc, int tri, int64_t tri_tiles) {
   __shared__ double As[2][BK * LDA_S];
   __shared__ double Bs[2][BK * LDB_S];
 
   const int64_t ntm = (m + BM - 1) / BM, ntn = (n + BN - 1) / BN;
   int64_t tm, tn;
-  tile_coords(blockIdx.x, ntm, ntn, tm, tn);
+  if (tri_tiles > 0) tri_tile_coords(blockIdx.x, tri_tiles, ntm, ntn, tri, tm, tn);
+  else tile_coords(blockIdx.x, ntm, ntn, tm, tn);
   const int64_t row0 = tm * BM, col0 = tn * BN;
   if (tri == kLower && row0 + BM - 1 < col0) return;   // tile strictly above the diagonal
   if (tri == kUpper && col0 + BN - 1 < row0) return;   // tile strictly below the diagonal
@@ -86,6 +125,11 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
       rb[q] = ok ? (TB ? B[j + l * ldb] : B[l + j * ldb]) : 0.0;
     }
   };
+  // Update form C <- C +- op(A) op(B) (beta 1, alpha +-1): the accumulators start
+  // from C, loaded with the first operand slab, and B is staged negated for
+  // alpha -1, so the epilogue only stores.
+  const bool cinit = beta == 1.0 && (alpha == 1.0 || alpha == -1.0);
+  const bool negb = cinit && alpha == -1.0;
   auto store_tiles = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -99,22 +143,39 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
       int j, l;
       if (TB) { j = t & 127; l = (t >> 7) + 2 * q; }
       else    { l = t & 15;  j = (t >> 4) + 16 * q; }
-      Bs[buf][l * LDB_S + j] = rb[q];
+      Bs[buf][l * LDB_S + j] = negb ? -rb[q] : rb[q];
     }
   };
 
+  const int lr = lane & 15, lk = lane >> 4;
+  // lane holds C[i0 + lr][j0 + lk + 4r] of each 16 x 16 tile
+  auto c_ok = [&](int64_t i, int64_t j) {
+    return i < m && j < n && !(tri == kLower && i < j) && !(tri == kUpper && i > j);
+  };
   d4 acc[4][4];
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 4; ++y) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+  if (cinit) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int64_t i = row0 + wm * 64 + 16 * x + lr;
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t j = col0 + wn * 64 + 16 * y + lk + 4 * r;
+          if (c_ok(i, j)) acc[x][y][r] = C[i + j * ldc];
+        }
+    }
+  }
 
   const int nk = (int)((k + BK - 1) / BK);
   load_tiles(0);
   store_tiles(0);
   __syncthreads();
 
-  const int lr = lane & 15, lk = lane >> 4;
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) load_tiles((int64_t)(kt + 1) * BK);
@@ -138,22 +199,46 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
     __syncthreads();
   }
 
-  // ---- epilogue: lane holds C[i0 + lr][j0 + lk + 4r] of each 16 x 16 tile
+  // ---- epilogue
+  if (cinit) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int64_t i = row0 + wm * 64 + 16 * x + lr;
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t j = col0 + wn * 64 + 16 * y + lk + 4 * r;
+          if (c_ok(i, j)) C[i + j * ldc] = acc[x][y][r];
+        }
+    }
+    return;
+  }
+  // general alpha, beta: per row tile the 16 C values are all requested before any
+  // is used (one memory latency per row tile instead of one per element)
+  const bool need_c = beta != 0.0;
 #pragma unroll
   for (int x = 0; x < 4; ++x) {
     const int64_t i = row0 + wm * 64 + 16 * x + lr;
-    if (i >= m) continue;
+    bool ok[4][4];
+    double cv[4][4];
 #pragma unroll
     for (int y = 0; y < 4; ++y) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t j = col0 + wn * 64 + 16 * y + lk + 4 * r;
-        if (j >= n) continue;
-        if (tri == kLower && i < j) continue;
-        if (tri == kUpper && i > j) continue;
-        double* cp = C + i + j * ldc;
+        ok[y][r] = i < m && j < n && !(tri == kLower && i < j) && !(tri == kUpper && i > j);
+        cv[y][r] = (need_c && ok[y][r]) ? C[i + j * ldc] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (!ok[y][r]) continue;
+        const int64_t j = col0 + wn * 64 + 16 * y + lk + 4 * r;
         const double v = alpha * acc[x][y][r];
-        *cp = beta == 0.0 ? v : (beta == 1.0 ? *cp + v : beta * *cp + v);
+        C[i + j * ldc] = beta == 0.0 ? v : (beta == 1.0 ? cv[y][r] + v : beta * cv[y][r] + v);
       }
     }
   }
@@ -192,12 +277,18 @@ void gemm_f64_cm(int transa, int transb, int64_t m, int64_t n, int64_t k, double
     return;
   }
   const bool ta = !is_notrans(transa), tb = !is_notrans(transb);
-  const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+  const int64_t ntm = (m + BM - 1) / BM, ntn = (n + BN - 1) / BN;
+  int64_t tiles = ntm * ntn, tri_tiles = 0;
+  if ((tri == kLower && ntm >= ntn) || (tri == kUpper && ntn >= ntm)) {
+    const int64_t nlong = tri == kLower ? ntm : ntn, nshort = tri == kLower ? ntn : ntm;
+    for (int64_t h = 0; 8 * h < nshort; ++h) tri_tiles += tri_group_tiles(h, nlong, nshort);
+    tiles = tri_tiles;
+  }
   dim3 grid((unsigned)tiles), block(NT);
-  if (!ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri);
-  else if (ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri);
-  else if (!ta && tb) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri);
-  else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri);
+  if (!ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
+  else if (ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
+  else if (!ta && tb) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
+  else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
   ZH_LAUNCH_CHECK();
 }
 

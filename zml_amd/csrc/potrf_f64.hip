@@ -3,12 +3,14 @@
 // zml's potrf is left-looking with NB = 64; on the GPU the same factor is
 // computed right-looking (the per-element operation order of the lower,
 // column-major case is the unblocked right-looking one either way, see the
-// oracle's closed form): for each 128-column block
-//   factor the diagonal block in one workgroup (LDS, separately rounded
+// oracle's closed form), blocked on two levels: for each 512-column block
+// column, in 128-column steps
+//   factor the diagonal block in one workgroup (registers, separately rounded
 //   multiply/subtract, l_ij = a_ij * fl(1/l_jj) as zml's right-side trsm
 //   does, correctly rounded sqrt),
-//   TRSM the block column below (lower) / block row right of it (upper),
-//   SYRK the trailing triangle on the MFMA GEMM.
+//   TRSM the rows below (lower) / columns right of it (upper),
+//   update the rest of the block column only;
+// then SYRK the trailing triangle with K = 512 on the MFMA GEMM.
 // Row-major lower is column-major upper of the same memory (and vice versa),
 // which is how row-major calls are mapped.
 //
@@ -120,25 +122,41 @@ potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* _
 
 void potrf_f64_cm(int uplo, int64_t n, double* a, int64_t lda, int* info, hipStream_t s) {
   const bool lower = uplo == kLower;
-  for (int64_t j0 = 0; j0 < n; j0 += DB) {
-    const int64_t jb = std::min<int64_t>(DB, n - j0);
-    double* d = a + j0 + j0 * lda;
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, (int)jb, d, lda, lower ? 1 : 0,
-                       info, j0);
-    ZH_LAUNCH_CHECK();
+  constexpr int64_t OB = 512;   // block column of the trailing SYRK (its K)
+  auto at = [&](int64_t i, int64_t j) { return a + i + j * lda; };
+  for (int64_t j0 = 0; j0 < n; j0 += OB) {
+    const int64_t jb = std::min<int64_t>(OB, n - j0);
+    // factor block column [j0, j0 + jb) in DB-wide steps whose updates stay inside it
+    for (int64_t k0 = j0; k0 < j0 + jb; k0 += DB) {
+      const int64_t kb = std::min<int64_t>(DB, j0 + jb - k0);
+      double* d = at(k0, k0);
+      hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, (int)kb, d, lda, lower ? 1 : 0, info, k0);
+      ZH_LAUNCH_CHECK();
+      const int64_t r = n - k0 - kb;           // rows below (lower) / columns right (upper)
+      if (r <= 0) break;
+      const int64_t w = j0 + jb - (k0 + kb);   // columns of the block column still to update
+      if (lower) {
+        // A21 <- A21 L11^-T ; the block column's part of A22 <- A22 - A21 A21^T
+        double* a21 = at(k0 + kb, k0);
+        trsm_f64_cm(kRight, kLower, kTrans, kNonUnit, r, kb, 1.0, d, lda, a21, lda, s);
+        if (w > 0)
+          gemm_f64_cm(kNoTrans, kTrans, r, w, kb, -1.0, a21, lda, a21, lda, 1.0, at(k0 + kb, k0 + kb), lda, s,
+                      kLower);
+      } else {
+        // A12 <- U11^-T A12 ; the block row's part of A22 <- A22 - A12^T A12
+        double* a12 = at(k0, k0 + kb);
+        trsm_f64_cm(kLeft, kUpper, kTrans, kNonUnit, kb, r, 1.0, d, lda, a12, lda, s);
+        if (w > 0)
+          gemm_f64_cm(kTrans, kNoTrans, w, r, kb, -1.0, a12, lda, a12, lda, 1.0, at(k0 + kb, k0 + kb), lda, s,
+                      kUpper);
+      }
+    }
     const int64_t r = n - j0 - jb;
     if (r <= 0) break;
-    if (lower) {
-      // A21 <- A21 L11^-T ; A22 <- A22 - A21 A21^T
-      double* a21 = a + (j0 + jb) + j0 * lda;
-      trsm_f64_cm(kRight, kLower, kTrans, kNonUnit, r, jb, 1.0, d, lda, a21, lda, s);
-      syrk_f64_cm(kLower, kNoTrans, r, jb, -1.0, a21, lda, 1.0, a + (j0 + jb) + (j0 + jb) * lda, lda, s);
-    } else {
-      // A12 <- U11^-T A12 ; A22 <- A22 - A12^T A12
-      double* a12 = a + j0 + (j0 + jb) * lda;
-      trsm_f64_cm(kLeft, kUpper, kTrans, kNonUnit, jb, r, 1.0, d, lda, a12, lda, s);
-      syrk_f64_cm(kUpper, kTrans, r, jb, -1.0, a12, lda, 1.0, a + (j0 + jb) + (j0 + jb) * lda, lda, s);
-    }
+    if (lower)
+      syrk_f64_cm(kLower, kNoTrans, r, jb, -1.0, at(j0 + jb, j0), lda, 1.0, at(j0 + jb, j0 + jb), lda, s);
+    else
+      syrk_f64_cm(kUpper, kTrans, r, jb, -1.0, at(j0, j0 + jb), lda, 1.0, at(j0 + jb, j0 + jb), lda, s);
   }
 }
 
