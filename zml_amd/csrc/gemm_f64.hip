@@ -16,6 +16,7 @@
 // hold 16 consecutive rows of one column of C: stores and beta-loads are
 // 128-byte contiguous in column-major memory.
 #include "zmlhip_internal.hpp"
+#include "tiles.hpp"
 
 namespace zmlhip {
 namespace {
@@ -45,44 +46,6 @@ __device__ __forceinline__ void tile_coords(int64_t bid, int64_t ntm, int64_t nt
   const int64_t r = pid - group * per_group;
   tm = first_m + r % gm;
   tn = r / gm;
-}
-
-// Tiles of a triangular update only (tri lower with ntm >= ntn, or upper with
-// ntn >= ntm): the active tiles are enumerated compactly, in groups of 8 tile
-// columns (lower; 8 tile rows for upper) so the group's B (A) panels stay in L2,
-// and split into equal contiguous ranges per XCD like tile_coords. In group h
-// (width g), the first g rows (columns) form a triangle of 1, 2, .., g tiles,
-// the rest are full rows (columns) of g tiles.
-__host__ __device__ inline int64_t tri_group_tiles(int64_t h, int64_t nlong, int64_t nshort) {
-  const int64_t g = nshort - 8 * h < 8 ? nshort - 8 * h : 8;
-  return g * (g + 1) / 2 + (nlong - 8 * h - g) * g;
-}
-
-__device__ __forceinline__ void tri_tile_coords(int64_t bid, int64_t nt, int64_t ntm, int64_t ntn, int tri,
-                                                int64_t& tm, int64_t& tn) {
-  const int64_t x = bid & 7, q = nt >> 3, r8 = nt & 7;
-  int64_t pid = x * q + (x < r8 ? x : r8) + (bid >> 3);
-  const int64_t nlong = tri == kLower ? ntm : ntn, nshort = tri == kLower ? ntn : ntm;
-  int64_t h = 0;
-  for (;; ++h) {
-    const int64_t c = tri_group_tiles(h, nlong, nshort);
-    if (pid < c) break;
-    pid -= c;
-  }
-  const int64_t g = nshort - 8 * h < 8 ? nshort - 8 * h : 8;
-  int64_t lo, sh;   // position along the long side, along the short side
-  if (pid < g * (g + 1) / 2) {
-    int64_t k = 0;
-    while ((k + 1) * (k + 2) / 2 <= pid) ++k;
-    lo = 8 * h + k;
-    sh = 8 * h + (pid - k * (k + 1) / 2);
-  } else {
-    const int64_t r = pid - g * (g + 1) / 2;
-    lo = 8 * h + g + r / g;
-    sh = 8 * h + r % g;
-  }
-  if (tri == kLower) { tm = lo; tn = sh; }
-  else { tm = sh; tn = lo; }
 }
 
 template <bool TA, bool TB>
@@ -278,12 +241,8 @@ void gemm_f64_cm(int transa, int transb, int64_t m, int64_t n, int64_t k, double
   }
   const bool ta = !is_notrans(transa), tb = !is_notrans(transb);
   const int64_t ntm = (m + BM - 1) / BM, ntn = (n + BN - 1) / BN;
-  int64_t tiles = ntm * ntn, tri_tiles = 0;
-  if ((tri == kLower && ntm >= ntn) || (tri == kUpper && ntn >= ntm)) {
-    const int64_t nlong = tri == kLower ? ntm : ntn, nshort = tri == kLower ? ntn : ntm;
-    for (int64_t h = 0; 8 * h < nshort; ++h) tri_tiles += tri_group_tiles(h, nlong, nshort);
-    tiles = tri_tiles;
-  }
+  const int64_t tri_tiles = tri_tile_count(tri, ntm, ntn);
+  const int64_t tiles = tri_tiles > 0 ? tri_tiles : ntm * ntn;
   dim3 grid((unsigned)tiles), block(NT);
   if (!ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
   else if (ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);

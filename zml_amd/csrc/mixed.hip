@@ -796,15 +796,24 @@ void tri_inv256(const float* L, int64_t ld, float* X, int64_t ldx, int unit, hip
 void tri_inverse_f32(int64_t kb, const float* L, int64_t ld, float* inv, int unit, hipStream_t s) {
   hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(kb * kb)), dim3(256), 0, s, kb, inv, kb);
   ZH_LAUNCH_CHECK();
-  if (kb == 256 || kb == 512) {
-    float* T = static_cast<float*>(workspace(sizeof(float) * 256 * 256, 17));
+  if (kb == 256) {
     tri_inv256(L, ld, inv, kb, unit, s);
-    if (kb == 512) {
-      tri_inv256(L + 256 * ld + 256, ld, inv + 256 * kb + 256, kb, unit, s);
-      mp_gemm_f32(256, 256, 256, 1.0f, L + 256 * ld, ld, inv, kb, 0.0f, nullptr, nullptr, T, 256, s);
-      mp_gemm_f32(256, 256, 256, -1.0f, inv + 256 * kb + 256, kb, T, 256, 0.0f, nullptr, nullptr, inv + 256 * kb,
-                  kb, s);
-    }
+    return;
+  }
+  if (kb == 512) {
+    // X22 on a second stream while X11 and T = L21 X11 run on s
+    float* T = static_cast<float*>(workspace(sizeof(float) * 256 * 256, 17));
+    const hipStream_t s2 = aux_stream(2);
+    const hipEvent_t ev_fork = aux_event(6), ev_join = aux_event(7);
+    ZH_CHECK(hipEventRecord(ev_fork, s));
+    ZH_CHECK(hipStreamWaitEvent(s2, ev_fork, 0));
+    tri_inv256(L + 256 * ld + 256, ld, inv + 256 * kb + 256, kb, unit, s2);
+    ZH_CHECK(hipEventRecord(ev_join, s2));
+    tri_inv256(L, ld, inv, kb, unit, s);
+    mp_gemm_f32(256, 256, 256, 1.0f, L + 256 * ld, ld, inv, kb, 0.0f, nullptr, nullptr, T, 256, s);
+    ZH_CHECK(hipStreamWaitEvent(s, ev_join, 0));
+    mp_gemm_f32(256, 256, 256, -1.0f, inv + 256 * kb + 256, kb, T, 256, 0.0f, nullptr, nullptr, inv + 256 * kb, kb,
+                s);
     return;
   }
   for (int64_t k0 = 0; k0 < kb; k0 += 64) {
@@ -1160,10 +1169,6 @@ __global__ void block_to_f32_kernel(int64_t kb, const double* __restrict__ src, 
   dst[i * ldd + j] = (float)(trans ? src[j * lds + i] : src[i * lds + j]);
 }
 
-__global__ void fill_identity_f64_kernel(int64_t kb, double* __restrict__ a) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e < kb * kb) a[e] = (e / kb == e % kb) ? 1.0 : 0.0;
-}
 
 // dst (cols x rows, ld ldd) = src^T (rows x cols, ld lds), 32 x 32 tiles
 __global__ void transpose_rect_f32_kernel(int64_t rows, int64_t cols, const float* __restrict__ src,
@@ -1507,7 +1512,6 @@ __global__ void colmax_kernel(int64_t n, const double* __restrict__ M, int64_t l
 
 struct CholWork {
   double* d64 = nullptr;       // diagonal block, FP64
-  double* inv64 = nullptr;     // its inverse
   int* info_blk = nullptr;     // potrf info of every diagonal block
   float* tmp = nullptr;        // A21 copy, (n + NB) x NB
   float* l21t[2] = {};         // L21^T for the FP32 update, NB x (n + NB), per block parity
@@ -1541,18 +1545,15 @@ void chol_factor(Solver& S, const CholWork& cw, int prec, double shift) {
     ZH_LAUNCH_CHECK();
     // row-major lower = column-major upper of the same memory
     potrf_f64_cm(kUpper, jb, cw.d64, jb, cw.info_blk + b, sp);
-    hipLaunchKernelGGL(fill_identity_f64_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb, cw.inv64);
-    ZH_LAUNCH_CHECK();
-    // (row-major) L11 X = I  <=>  (column-major) X' U = I with U the column-major view of L11
-    trsm_f64_cm(kRight, kUpper, kNoTrans, kNonUnit, jb, jb, 1.0, cw.d64, jb, cw.inv64, jb, sp);
     float* invL = S.invL + b * NB_MIX * NB_MIX;
     float* invLT = S.invLT + b * NB_MIX * NB_MIX;
     hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb, (const double*)cw.d64, jb,
                        Wd, n, 0, 1);
-    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb,
-                       (const double*)cw.inv64, jb, invL, jb, 0, 0);
-    hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb,
-                       (const double*)cw.inv64, jb, invLT, jb, 1, 0);
+    ZH_LAUNCH_CHECK();
+    // the FP32 inverse of the rounded factor (as the LU panels invert U11), and its transpose
+    tri_inverse_f32(jb, Wd, n, invL, 0, sp);
+    hipLaunchKernelGGL(transpose_rect_f32_kernel, dim3(blocks(jb, 32), blocks(jb, 32)), dim3(32, 8), 0, sp, jb, jb,
+                       (const float*)invL, jb, invLT, jb);
     ZH_LAUNCH_CHECK();
     if (m2 > 0) {
       float* A21 = S.W + (j0 + jb) * n + j0;
@@ -1945,7 +1946,7 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
   char* small = static_cast<char*>(workspace(
       sizeof(float) * 2 * S.nblk * blk2 + sizeof(double) * 2 * n + sizeof(double) * 512 +
           sizeof(float) * (n + NB_MIX) * MAXR + sizeof(uint64_t) * (n + NB_MIX) * MAXR +
-          sizeof(double) * MAXR * n * ssp.splits + sizeof(double) * 2 * blk2 + sizeof(int) * (S.nblk + 64) +
+          sizeof(double) * MAXR * n * ssp.splits + sizeof(double) * blk2 + sizeof(int) * (S.nblk + 64) +
           sizeof(double) * 2 * MAXR + 16 * 256, 9));
   auto carve = [&](size_t bytes) { char* p = small; small += (bytes + 255) / 256 * 256; return p; };
   S.invL = reinterpret_cast<float*>(carve(sizeof(float) * S.nblk * blk2));
@@ -1977,7 +1978,6 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
   }
   CholWork cw;
   cw.d64 = reinterpret_cast<double*>(carve(sizeof(double) * blk2));
-  cw.inv64 = reinterpret_cast<double*>(carve(sizeof(double) * blk2));
   cw.info_blk = reinterpret_cast<int*>(carve(sizeof(int) * (S.nblk + 64)));
   double* cmax = reinterpret_cast<double*>(carve(sizeof(double) * 2 * MAXR));
   char* lp = static_cast<char*>(workspace(sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX * 3 +

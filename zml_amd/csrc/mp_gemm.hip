@@ -17,6 +17,7 @@
 // 8 (r / 4) + 4 ((l / 32) % 2) + r % 4 of register r, so every C access is
 // 32 consecutive floats.
 #include "zmlhip_internal.hpp"
+#include "tiles.hpp"
 #include "mixed.hpp"
 
 namespace zmlhip {
@@ -102,13 +103,14 @@ template <typename H>
 __global__ void __launch_bounds__(NTH, 2)
 mp_gemm_h_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restrict__ A, int64_t lda,
                  const H* __restrict__ BT, int64_t ldb, float beta, const float* __restrict__ rs,
-                 const float* __restrict__ cs, float* __restrict__ C, int64_t ldc, int lower) {
+                 const float* __restrict__ cs, float* __restrict__ C, int64_t ldc, int lower, int64_t tri_tiles) {
   typedef typename MfmaH<H>::vec V;
   __shared__ __attribute__((aligned(16))) H As[2][MT * LDH];
   __shared__ __attribute__((aligned(16))) H Bs[2][NTL * LDH];
   const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
   int64_t tm, tn;
-  tile_of(blockIdx.x, ntm, ntn, tm, tn);
+  if (tri_tiles > 0) tri_tile_coords(blockIdx.x, tri_tiles, ntm, ntn, kLower, tm, tn);
+  else tile_of(blockIdx.x, ntm, ntn, tm, tn);
   const int64_t row0 = tm * MT, col0 = tn * NTL;
   if (lower && col0 > row0 + MT - 1) return;   // tile strictly above the diagonal
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -201,12 +203,13 @@ __global__ void __launch_bounds__(NTH, 2)
 mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __restrict__ A,
                  int64_t lda, const float* __restrict__ B, int64_t ldb, float beta,
                  const float* __restrict__ rs, const float* __restrict__ cs, float* __restrict__ C,
-                 int64_t ldc, int lower) {
+                 int64_t ldc, int lower, int64_t tri_tiles) {
   __shared__ float As[2][MT * LDAF];
   __shared__ float Bs[2][BKF * LDBF];
   const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
   int64_t tm, tn;
-  tile_of(blockIdx.x, ntm, ntn, tm, tn);
+  if (tri_tiles > 0) tri_tile_coords(blockIdx.x, tri_tiles, ntm, ntn, kLower, tm, tn);
+  else tile_of(blockIdx.x, ntm, ntn, tm, tn);
   const int64_t row0 = tm * MT, col0 = tn * NTL;
   if (lower && col0 > row0 + MT - 1) return;   // tile strictly above the diagonal
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -282,9 +285,11 @@ void mp_gemm_f16(int64_t m, int64_t n, int64_t K, float alpha, const _Float16* A
                  const _Float16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
                  float* C, int64_t ldc, hipStream_t s, int lower) {
   if (m <= 0 || n <= 0) return;
-  const int64_t tiles = ((m + MT - 1) / MT) * ((n + NTL - 1) / NTL);
+  const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
+  const int64_t tri_tiles = lower ? tri_tile_count(kLower, ntm, ntn) : 0;
+  const int64_t tiles = tri_tiles > 0 ? tri_tiles : ntm * ntn;
   hipLaunchKernelGGL((mp_gemm_h_kernel<_Float16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
-                     A, lda, BT, ldb, beta, rs, cs, C, ldc, lower);
+                     A, lda, BT, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
   ZH_LAUNCH_CHECK();
 }
 
@@ -292,9 +297,11 @@ void mp_gemm_bf16(int64_t m, int64_t n, int64_t K, float alpha, const __bf16* A,
                   const __bf16* BT, int64_t ldb, float beta, const float* rs, const float* cs,
                   float* C, int64_t ldc, hipStream_t s, int lower) {
   if (m <= 0 || n <= 0) return;
-  const int64_t tiles = ((m + MT - 1) / MT) * ((n + NTL - 1) / NTL);
+  const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
+  const int64_t tri_tiles = lower ? tri_tile_count(kLower, ntm, ntn) : 0;
+  const int64_t tiles = tri_tiles > 0 ? tri_tiles : ntm * ntn;
   hipLaunchKernelGGL((mp_gemm_h_kernel<__bf16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
-                     A, lda, BT, ldb, beta, rs, cs, C, ldc, lower);
+                     A, lda, BT, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
   ZH_LAUNCH_CHECK();
 }
 
@@ -302,9 +309,11 @@ void mp_gemm_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, i
                  const float* B, int64_t ldb, float beta, const float* rs, const float* cs, float* C,
                  int64_t ldc, hipStream_t s, int lower) {
   if (m <= 0 || n <= 0) return;
-  const int64_t tiles = ((m + MT - 1) / MT) * ((n + NTL - 1) / NTL);
+  const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
+  const int64_t tri_tiles = lower ? tri_tile_count(kLower, ntm, ntn) : 0;
+  const int64_t tiles = tri_tiles > 0 ? tri_tiles : ntm * ntn;
   hipLaunchKernelGGL(mp_gemm_f_kernel, dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha, A, lda,
-                     B, ldb, beta, rs, cs, C, ldc, lower);
+                     B, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
   ZH_LAUNCH_CHECK();
 }
 
