@@ -159,8 +159,8 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
   hipStream_t sp = aux_stream(0);
   hipEvent_t ev_in = aux_event(0), ev_panel = aux_event(1), ev_next = aux_event(2);
   const hipStream_t caller = s;
-  if (ops.isolate_panel) {   // the trailing updates move to the CU-sparing stream
-    s = stream_sparing_cus();
+  if (ops.spare_cus > 0) {   // the trailing updates move to a CU-sparing stream
+    s = stream_sparing_cus(ops.spare_cus);
     ZH_CHECK(hipEventRecord(ev_in, caller));
     ZH_CHECK(hipStreamWaitEvent(s, ev_in, 0));
   }
@@ -244,8 +244,19 @@ static void apply_inv_f64(int64_t kb, int64_t cols, const double* inv, double* B
 
 void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, int* info,
                   hipStream_t s) {
+  // The pivot leaf keeps one 256-row workgroup per CU resident for the whole
+  // panel; when its first grid fits in a quarter of the CUs, those CUs are kept
+  // free of trailing updates so the leaf never waits for GEMM tiles to drain.
+  // (rounded up to a power of two from 8, so few distinct masked streams exist)
+  const int leaf_wgs = (int)((m + panel::pt_of<double>() - 1) / panel::pt_of<double>());
+  int spare = 0;
+  if (leaf_wgs > 1) {
+    spare = 8;
+    while (spare < leaf_wgs) spare *= 2;
+    if (spare > device_cu_count() / 4) spare = 0;
+  }
   const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64, nullptr, nullptr,
-                          nullptr, false};
+                          nullptr, spare};
   getrf_lookahead<double>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }
 

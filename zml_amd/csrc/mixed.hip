@@ -1789,7 +1789,7 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
     static float* at(int64_t j0) { return tl_mix.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX; }
   };
   const LuOps<float> ops{trailing_gemm, trsm_f32, Inv::store, apply_inv_f32, panel_gemm_f32, try_panel_f32,
-                         InvAt::at, true};
+                         InvAt::at, 8};
   getrf_lookahead<float>(n, n, S.W, n, S.ipiv, S.info, NB_MIX, ops, s);
   int info = 0;
   ZH_CHECK(hipMemcpyAsync(&info, S.info, sizeof(int), hipMemcpyDeviceToHost, s));

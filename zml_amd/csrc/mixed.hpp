@@ -45,9 +45,9 @@ struct LuOps {
   // optional: where L11^-1 of the block column starting at j0 is kept (the
   // driver's two alternating buffers when null)
   T* (*inv_store)(int64_t j0);
-  // run the trailing updates on a stream that spares a few CUs, so that the
-  // panel stream's single-workgroup kernels always find a free CU
-  bool isolate_panel;
+  // > 0: run the trailing updates on a stream that leaves this many CUs free,
+  // so that the panel stream's workgroups always find empty CUs
+  int spare_cus;
 };
 
 int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, int64_t lda,

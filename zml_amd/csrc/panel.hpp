@@ -332,42 +332,54 @@ constexpr int PERM_MAX = 1024;  // pairs (2 x block of 512 interchanges)
 
 // Composes the interchanges i <-> ipiv[i]-1 for i = k1..k2 (ascending, every
 // ipiv[i]-1 >= i) into a list of moves dst <- src over the rows touched.
-// One wave. pairs[0] = count, then (dst, src) pairs.
+// One wave: the lanes stage the pivots and clear an open-addressing table
+// (row -> slot) of the rows beyond the block, then one lane walks the
+// interchanges in order, each a table probe and a swap in LDS.
+// pairs[0] = count, then (dst, src) pairs.
+constexpr int PERM_HT = 2 * PERM_MAX;   // power of two, load factor <= 1/4
+static_assert((PERM_HT & (PERM_HT - 1)) == 0, "table size must be a power of two");
 __global__ void compose_perm_kernel(int k1, int k2, const int32_t* __restrict__ ipiv,
                                     int32_t* __restrict__ pairs) {
-  __shared__ int32_t pos[PERM_MAX], cont[PERM_MAX];
+  __shared__ int32_t pos[PERM_MAX], cont[PERM_MAX], piv[PERM_MAX / 2];
+  __shared__ int32_t hkey[PERM_HT], hslot[PERM_HT];
   __shared__ int s_n;
   const int lane = threadIdx.x;
   const int kb = k2 - k1 + 1;
-  for (int q = lane; q < kb; q += 64) { pos[q] = k1 + q; cont[q] = k1 + q; }
-  if (lane == 0) s_n = kb;
+  for (int q = lane; q < kb; q += 64) { pos[q] = k1 + q; cont[q] = k1 + q; piv[q] = ipiv[k1 + q] - 1; }
+  for (int q = lane; q < PERM_HT; q += 64) hkey[q] = -1;
   __syncthreads();
-  for (int i = k1; i <= k2; ++i) {
-    const int p = ipiv[i] - 1;
-    int qp;
-    if (p <= k2) {
-      qp = p - k1;
-    } else {
-      const int n = s_n;
-      int found = INT32_MAX;
-      for (int q = kb + lane; q < n; q += 64)
-        if (pos[q] == p) found = q;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) found = min(found, __shfl_xor(found, off));
-      if (found == INT32_MAX) {
-        found = n;
-        if (lane == 0) { pos[n] = p; cont[n] = p; s_n = n + 1; }
+  if (lane == 0) {
+    int n = kb;
+    for (int i = 0; i < kb; ++i) {
+      const int p = piv[i];
+      int qp;
+      if (p <= k2) {
+        qp = p - k1;
+      } else {
+        uint32_t h = ((uint32_t)p * 2654435761u) & (PERM_HT - 1);
+        for (;;) {
+          const int key = hkey[h];
+          if (key == p) { qp = hslot[h]; break; }
+          if (key < 0) {
+            hkey[h] = p;
+            hslot[h] = n;
+            pos[n] = p;
+            cont[n] = p;
+            qp = n++;
+            break;
+          }
+          h = (h + 1) & (PERM_HT - 1);
+        }
       }
-      qp = found;
+      if (qp != i) {
+        const int tmp = cont[i];
+        cont[i] = cont[qp];
+        cont[qp] = tmp;
+      }
     }
-    __syncthreads();
-    if (lane == 0 && qp != i - k1) {
-      const int tmp = cont[i - k1];
-      cont[i - k1] = cont[qp];
-      cont[qp] = tmp;
-    }
-    __syncthreads();
+    s_n = n;
   }
+  __syncthreads();
   const int n = s_n;
   if (lane == 0) {
     int c = 0;

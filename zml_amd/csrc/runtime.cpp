@@ -2,7 +2,9 @@
 // synchronous host-pointer API and growable device workspaces.
 #include "zmlhip_internal.hpp"
 
+#include <algorithm>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -18,12 +20,13 @@ struct Buf {
 };
 constexpr int kSlots = 24;
 constexpr int kAux = 4, kEvents = 8;
-constexpr int kReservedCus = 8;   // CUs a sparing stream leaves to the others
+constexpr int kSparing = 8;   // CU-sparing streams kept per device (distinct reservations)
 struct DeviceState {
   Buf slot[kSlots];
   hipStream_t stream = nullptr;
   hipStream_t aux[kAux] = {};
-  hipStream_t reserved = nullptr;   // every CU but the first kReservedCus
+  int reserved_cus[kSparing] = {};
+  hipStream_t reserved[kSparing] = {};   // every CU but the first reserved_cus[i]
   hipEvent_t ev[kEvents] = {};
 };
 struct ThreadState {
@@ -38,7 +41,8 @@ struct ThreadState {
       if (kv.second.stream) (void)hipStreamDestroy(kv.second.stream);
       for (auto& st : kv.second.aux)
         if (st) (void)hipStreamDestroy(st);
-      if (kv.second.reserved) (void)hipStreamDestroy(kv.second.reserved);
+      for (auto& st : kv.second.reserved)
+        if (st) (void)hipStreamDestroy(st);
       for (auto& e : kv.second.ev)
         if (e) (void)hipEventDestroy(e);
       (void)hipSetDevice(cur);
@@ -139,15 +143,21 @@ hipStream_t aux_stream(int idx) {
   return st;
 }
 
-hipStream_t stream_sparing_cus() {
+hipStream_t stream_sparing_cus(int reserved) {
   DeviceState& ds = current();
-  if (!ds.reserved) {
-    const int cus = device_cu_count();
+  const int cus = device_cu_count();
+  reserved = std::max(1, std::min(reserved, cus - 1));
+  for (int i = 0; i < kSparing; ++i)
+    if (ds.reserved[i] && ds.reserved_cus[i] == reserved) return ds.reserved[i];
+  for (int i = 0; i < kSparing; ++i) {
+    if (ds.reserved[i]) continue;
     std::vector<uint32_t> mask((cus + 31) / 32, 0u);
-    for (int c = kReservedCus; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
-    ZH_CHECK(hipExtStreamCreateWithCUMask(&ds.reserved, (uint32_t)mask.size(), mask.data()));
+    for (int c = reserved; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+    ZH_CHECK(hipExtStreamCreateWithCUMask(&ds.reserved[i], (uint32_t)mask.size(), mask.data()));
+    ds.reserved_cus[i] = reserved;
+    return ds.reserved[i];
   }
-  return ds.reserved;
+  throw std::runtime_error("stream_sparing_cus: too many distinct reservations");
 }
 
 hipEvent_t aux_event(int idx) {

@@ -88,9 +88,9 @@ hipStream_t host_api_stream();
 // on the current device, used to overlap panel factorization with updates.
 hipStream_t aux_stream(int idx);
 hipEvent_t aux_event(int idx);
-// A library stream that leaves a few CUs free for latency-bound work on other
-// streams (one per device and host thread).
-hipStream_t stream_sparing_cus();
+// A library stream whose kernels leave the first `reserved` CUs free for
+// latency-bound work on other streams (one per reservation, device and host thread).
+hipStream_t stream_sparing_cus(int reserved = 8);
 
 // Optional per-launch timing of the dominant kernels (kinds below), with HIP
 // events recorded on the stream each kernel is launched on.
