@@ -840,11 +840,22 @@ void apply_inv_f32(int64_t kb, int64_t cols, const float* inv, float* B, int64_t
 // taken from src21 (rows - DB x DB, ld src_ld; P's own rows below the block
 // are overwritten). U11^-1 goes to uinv (ld uinv_ld). *pfail records a zero
 // pivot (1) or an |l| above the threshold (2).
+// linv (optional): the inverse of the block's unit lower factor is formed there
+// (its strictly upper part already zero) on a second stream, joined by the
+// event aux_event(5), while U11^-1 and L21 take this stream.
 void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, int64_t src_ld, int32_t* ipiv,
-                      int base, float* uinv, int64_t uinv_ld, hipStream_t s) {
+                      int base, float* uinv, int64_t uinv_ld, hipStream_t s, float* linv = nullptr,
+                      int64_t linv_ld = 0) {
   MixedCtx& mx = tl_mix;
   hipLaunchKernelGGL(diag_getf2_kernel<DIAG_TY>, dim3(1), dim3(DIAG_TY * 32), 0, s, P, lda, ipiv, base, mx.pfail);
   ZH_LAUNCH_CHECK();
+  if (linv) {
+    const hipStream_t s2 = aux_stream(2);
+    ZH_CHECK(hipEventRecord(aux_event(4), s));
+    ZH_CHECK(hipStreamWaitEvent(s2, aux_event(4), 0));
+    tri_inv256(P, lda, linv, linv_ld, 1, s2);
+    ZH_CHECK(hipEventRecord(aux_event(5), s2));
+  }
   // U11^-1 (upper) into scr1, whose strictly lower part must be zero
   hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr1,
                      (int64_t)DB);
@@ -883,17 +894,20 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
   ZH_CHECK(hipMemsetAsync(mx.pfail, 0, sizeof(int), s));
   ZH_CHECK(hipMemcpy2DAsync(mx.pbak, sizeof(float) * jb, P, sizeof(float) * lda, sizeof(float) * jb, rows,
                             hipMemcpyDeviceToDevice, s));
-  optimistic_block(P, lda, rows, mx.pbak + DB * jb, jb, ipiv + j0, (int)j0, ustore, NB_MIX, s);
+  // L11^-1 (512 x 512, kept for the driver's U12): La^-1 and Lb^-1 beside the
+  // halves' other work, the off-diagonal block at the end
+  float* linv = jb == 2 * DB ? mx.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX : nullptr;
+  if (linv) {
+    hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, linv, jb);
+    ZH_LAUNCH_CHECK();
+  }
+  optimistic_block(P, lda, rows, mx.pbak + DB * jb, jb, ipiv + j0, (int)j0, ustore, NB_MIX, s, linv, jb);
   if (jb == 2 * DB) {
     const int64_t c2 = j0 + DB;                 // second half: columns c2 .. c2 + DB
     const int64_t rows2 = rows - DB;
     const int32_t* pr = compose_perm((int)j0, (int)(c2 - 1), ipiv, s, 2);
     apply_perm<float>(DB, A + c2, lda, pr, s);
-    // L11^-1 (512 x 512, kept for the driver's U12): La^-1 now, the rest below
-    float* linv = mx.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;
-    hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, linv, jb);
-    ZH_LAUNCH_CHECK();
-    tri_inv256(P, lda, linv, jb, 1, s);
+    ZH_CHECK(hipStreamWaitEvent(s, aux_event(5), 0));   // La^-1
     // U12 of the first half = La^-1 A12, then A22 -= L21 U12 within the block column
     ZH_CHECK(hipMemcpy2DAsync(mx.scr4, sizeof(float) * DB, A + j0 * lda + c2, sizeof(float) * lda,
                               sizeof(float) * DB, DB, hipMemcpyDeviceToDevice, s));
@@ -904,11 +918,11 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
       ZH_CHECK(hipMemcpy2DAsync(mx.ptmp, sizeof(float) * DB, A + (c2 + DB) * lda + c2, sizeof(float) * lda,
                                 sizeof(float) * DB, rows2 - DB, hipMemcpyDeviceToDevice, s));
     optimistic_block(A + c2 * lda + c2, lda, rows2, mx.ptmp, DB, ipiv + c2, (int)c2,
-                     ustore + DB * NB_MIX + DB, NB_MIX, s);
+                     ustore + DB * NB_MIX + DB, NB_MIX, s, linv + DB * jb + DB, jb);
     const int32_t* pr2 = compose_perm((int)c2, (int)(c2 + DB - 1), ipiv, s, 2);
     apply_perm<float>(DB, A + j0, lda, pr2, s);
-    // Lb^-1, then the off-diagonal block -Lb^-1 Lba La^-1
-    tri_inv256(A + c2 * lda + c2, lda, linv + DB * jb + DB, jb, 1, s);
+    // the off-diagonal block -Lb^-1 Lba La^-1
+    ZH_CHECK(hipStreamWaitEvent(s, aux_event(5), 0));   // Lb^-1
     mp_gemm_f32(DB, DB, DB, 1.0f, A + c2 * lda + j0, lda, linv, jb, 0.0f, nullptr, nullptr, mx.scr3, DB, s);
     mp_gemm_f32(DB, DB, DB, -1.0f, linv + DB * jb + DB, jb, mx.scr3, DB, 0.0f, nullptr, nullptr, linv + DB * jb,
                 jb, s);
