@@ -194,6 +194,118 @@ mp_gemm_h_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restri
   epilogue(acc, cv, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
 }
 
+// Same product with the operand slabs streamed straight into LDS
+// (global_load_lds_dwordx4: a wave writes 1 KB, 16 rows of 64 bytes, lane l
+// the 16 bytes at 16 l) through a ring of S stages: slab kt + S - 1 is
+// requested while slab kt is multiplied, one barrier per slab and no
+// registers for staging. Within a row the four 16-byte k-chunks are stored
+// XOR-swizzled by (row >> 1) & 3 (each lane picks the global chunk that lands
+// at its position), so the MFMA operand reads of 8 consecutive rows hit
+// distinct banks. Needs K % 32 == 0 and 16-byte aligned rows; rows beyond m
+// (columns beyond n) read the last valid row, whose products land only in C
+// entries that are not stored. Operands and products are those of
+// mp_gemm_h_kernel, in the same order.
+constexpr int DL_STAGES = 4;
+__device__ __forceinline__ int dl_swz(int r) { return (r >> 1) & 3; }
+
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  // s_waitcnt vmcnt(N), the other counters unconstrained (gfx9 encoding)
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+template <typename H>
+__global__ void __launch_bounds__(NTH, 2)
+mp_gemm_h_dl_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restrict__ A, int64_t lda,
+                    const H* __restrict__ BT, int64_t ldb, float beta, const float* __restrict__ rs,
+                    const float* __restrict__ cs, float* __restrict__ C, int64_t ldc, int lower,
+                    int64_t tri_tiles) {
+  typedef typename MfmaH<H>::vec V;
+  constexpr int S = DL_STAGES;
+  constexpr int ROWB = 64;                 // bytes per LDS row: 32 halfs
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[S][2][MT * ROWB];
+  const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
+  int64_t tm, tn;
+  if (tri_tiles > 0) tri_tile_coords(blockIdx.x, tri_tiles, ntm, ntn, kLower, tm, tn);
+  else tile_of(blockIdx.x, ntm, ntn, tm, tn);
+  const int64_t row0 = tm * MT, col0 = tn * NTL;
+  if (lower && col0 > row0 + MT - 1) return;   // tile strictly above the diagonal
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // wave w requests rows 32 w .. 32 w + 31 of both operands, two 16-row instructions each
+  const H* ga[2];
+  const H* gb[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int r = 32 * wave + 16 * j + (lane >> 2);
+    const int c = (lane & 3) ^ dl_swz(r);
+    const int64_t ia = row0 + r < m ? row0 + r : m - 1;
+    const int64_t jb = col0 + r < n ? col0 + r : n - 1;
+    ga[j] = A + ia * lda + 8 * c;
+    gb[j] = BT + jb * ldb + 8 * c;
+  }
+  auto issue = [&](int kt, int st) {
+    const int64_t ko = (int64_t)kt * BKH;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      __builtin_amdgcn_global_load_lds((const void*)(ga[j] + ko),
+                                       (__attribute__((address_space(3))) void*)&lds[st][0][(32 * wave + 16 * j) * ROWB],
+                                       16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(gb[j] + ko),
+                                       (__attribute__((address_space(3))) void*)&lds[st][1][(32 * wave + 16 * j) * ROWB],
+                                       16, 0, 0);
+    }
+  };
+
+  f16v acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+  f16v cv[2][2];
+  load_c(cv, m, n, beta, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
+
+  const int nk = (int)(K / BKH);
+#pragma unroll
+  for (int s0 = 0; s0 < S - 1; ++s0)
+    if (s0 < nk) issue(s0, s0);
+  const int li = lane & 31, hk = lane >> 5;
+  for (int kt = 0; kt < nk; ++kt) {
+    // this wave's part of slab kt has landed once at most the later slabs' requests are pending
+    if (kt + S - 2 < nk) wait_vm<4 * (S - 2)>();
+    else wait_vm<0>();
+    // every wave's part of slab kt is in LDS and stage (kt - 1) % S is no longer read:
+    // a bare barrier (the fence of __syncthreads would wait for every request in flight)
+    asm volatile("s_barrier" ::: "memory");
+    if (kt + S - 1 < nk) issue(kt + S - 1, (kt + S - 1) % S);
+    const int st = kt % S;
+#pragma unroll
+    for (int ks = 0; ks < BKH / 16; ++ks) {
+      const int c = 2 * ks + hk;   // logical 16-byte chunk of the MFMA operand
+      V av[2], bv[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int r = wm * 64 + 32 * x + li;
+        av[x] = *reinterpret_cast<const V*>(&lds[st][0][r * ROWB + 16 * (c ^ dl_swz(r))]);
+      }
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        const int r = wn * 64 + 32 * y + li;
+        bv[y] = *reinterpret_cast<const V*>(&lds[st][1][r * ROWB + 16 * (c ^ dl_swz(r))]);
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = MfmaH<H>::mma(av[x], bv[y], acc[x][y]);
+    }
+  }
+  epilogue(acc, cv, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
+}
+
 // ---- FP32 ------------------------------------------------------------------------
 constexpr int BKF = 16;
 constexpr int LDAF = BKF + 1;        // A rows in LDS: [i][k], padded
@@ -279,6 +391,12 @@ mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __re
   epilogue(acc, cv, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
 }
 
+// the streamed variant applies: whole 32-wide k-slabs, 16-byte aligned rows
+bool dl_ok(int64_t K, const void* A, int64_t lda, const void* BT, int64_t ldb) {
+  return K >= BKH && K % BKH == 0 && lda % 8 == 0 && ldb % 8 == 0 &&
+         (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(BT) & 15) == 0;
+}
+
 }  // namespace
 
 void mp_gemm_f16(int64_t m, int64_t n, int64_t K, float alpha, const _Float16* A, int64_t lda,
@@ -288,8 +406,12 @@ void mp_gemm_f16(int64_t m, int64_t n, int64_t K, float alpha, const _Float16* A
   const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
   const int64_t tri_tiles = lower ? tri_tile_count(kLower, ntm, ntn) : 0;
   const int64_t tiles = tri_tiles > 0 ? tri_tiles : ntm * ntn;
-  hipLaunchKernelGGL((mp_gemm_h_kernel<_Float16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
-                     A, lda, BT, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
+  if (dl_ok(K, A, lda, BT, ldb))
+    hipLaunchKernelGGL((mp_gemm_h_dl_kernel<_Float16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
+                       A, lda, BT, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
+  else
+    hipLaunchKernelGGL((mp_gemm_h_kernel<_Float16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
+                       A, lda, BT, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
   ZH_LAUNCH_CHECK();
 }
 
@@ -300,8 +422,12 @@ void mp_gemm_bf16(int64_t m, int64_t n, int64_t K, float alpha, const __bf16* A,
   const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
   const int64_t tri_tiles = lower ? tri_tile_count(kLower, ntm, ntn) : 0;
   const int64_t tiles = tri_tiles > 0 ? tri_tiles : ntm * ntn;
-  hipLaunchKernelGGL((mp_gemm_h_kernel<__bf16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
-                     A, lda, BT, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
+  if (dl_ok(K, A, lda, BT, ldb))
+    hipLaunchKernelGGL((mp_gemm_h_dl_kernel<__bf16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
+                       A, lda, BT, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
+  else
+    hipLaunchKernelGGL((mp_gemm_h_kernel<__bf16>), dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha,
+                       A, lda, BT, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
   ZH_LAUNCH_CHECK();
 }
 
