@@ -146,8 +146,8 @@ int zmlhip_batch_plan_create(zmlhip_batch_plan **plan, int count, const int *n, 
                              double *const *x, const zmlhip_ir_opts *opts, int autotune, void *stream);
 int zmlhip_batch_plan_execute(zmlhip_batch_plan *plan, void *stream);
 int zmlhip_batch_plan_status(zmlhip_batch_plan *plan, int *status, void *stream);
-/* thread-grid side, values per thread side and system count of the 4 size
- * buckets (n <= 32, 64, 128, 256) */
+/* thread rows of the grid, matrix rows per thread (their product bounds the
+ * bucket's n) and system count of the 4 size buckets (n <= 32, 64, 128, 256) */
 int zmlhip_batch_plan_shapes(const zmlhip_batch_plan *plan, int *tg, int *r, int *count);
 int zmlhip_batch_plan_destroy(zmlhip_batch_plan *plan);
 

@@ -7,7 +7,7 @@
 // A plan sorts the systems into size buckets (n <= 32, 64, 128, 256). Every
 // bucket runs one persistent kernel whose workgroups take systems from an
 // atomic work queue; a workgroup keeps its system's FP32 factor in registers
-// (a TG x TG thread grid, R x R values per thread, NMAX = TG R) from the
+// (a TY x TX thread grid, RI x RJ values per thread, NMAX = TY RI = TX RJ) from the
 // factorization through every refinement step, and reads the FP64 matrix
 // from HBM only for the residuals. The thread-grid shape of every bucket is
 // chosen by timing the candidate shapes on the bucket's own systems
@@ -75,15 +75,16 @@ __device__ __forceinline__ double block_max(double v, double* red, int t) {
   return m;
 }
 
-template <int TG, int R>
-__global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
-  constexpr int NT = TG * TG, NMAX = TG * R, NW = (NT + 63) / 64;
+template <int TY, int TX, int RI, int RJ>
+__global__ void __launch_bounds__(TY * TX) batched_solve_kernel(BatchArgs g) {
+  constexpr int NT = TY * TX, NMAX = TY * RI, NW = (NT + 63) / 64;
+  static_assert(TX * RJ == NMAX && TX <= 64 && RI <= 32, "thread grid shape");
   __shared__ uint64_t wbest[NW];
   __shared__ float lcol[NMAX], urow[NMAX], cv[NMAX], yv[NMAX];
   __shared__ int prow[NMAX];
   __shared__ double xv[NMAX], rv[NMAX], red[NW];
   __shared__ int s_idx, s_bad;
-  const int t = threadIdx.x, tx = t % TG, ty = t / TG, lane = t & 63, wv = t >> 6;
+  const int t = threadIdx.x, tx = t % TX, ty = t / TX, lane = t & 63, wv = t >> 6;
   for (;;) {
     if (t == 0) s_idx = atomicAdd(g.counter, 1);
     __syncthreads();
@@ -95,37 +96,37 @@ __global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
     const int n = d.n;
 
     // ---- FP32 Gauss-Jordan inversion, in registers
-    float a[R][R];
+    float a[RI][RJ];
 #pragma unroll
-    for (int i = 0; i < R; ++i)
+    for (int i = 0; i < RI; ++i)
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const int row = ty + TG * i, col = tx + TG * j;
+      for (int j = 0; j < RJ; ++j) {
+        const int row = ty + TY * i, col = tx + TX * j;
         a[i][j] = (row < n && col < n) ? (float)d.a[row + (int64_t)col * n] : 0.0f;
       }
     unsigned done = 0;
 #pragma unroll
-    for (int i = 0; i < R; ++i)
-      if (ty + TG * i >= n) done |= 1u << i;
+    for (int i = 0; i < RI; ++i)
+      if (ty + TY * i >= n) done |= 1u << i;
     if (t == 0) s_bad = 0;
 #pragma unroll
-    for (int jj = 0; jj < R; ++jj) {
-      for (int kq = 0; kq < TG; ++kq) {
-        const int k = jj * TG + kq;
+    for (int jj = 0; jj < RJ; ++jj) {
+      for (int kq = 0; kq < TX; ++kq) {
+        const int k = jj * TX + kq;
         if (k >= n) break;
         int p = k;
         if (!d.spd) {
           uint64_t best = 0;
           if (tx == kq) {
 #pragma unroll
-            for (int i = 0; i < R; ++i)
+            for (int i = 0; i < RI; ++i)
               if (!((done >> i) & 1u)) {
-                const uint64_t key = bkey(fabsf(a[i][jj]), ty + TG * i);
+                const uint64_t key = bkey(fabsf(a[i][jj]), ty + TY * i);
                 best = key > best ? key : best;
               }
           }
 #pragma unroll
-          for (int off = TG; off < 64; off <<= 1) {
+          for (int off = TX; off < 64; off <<= 1) {
             const uint64_t o = __shfl_xor(best, off);
             best = o > best ? o : best;
           }
@@ -139,45 +140,45 @@ __global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
           for (int w = 1; w < NW; ++w) gk = wbest[w] > gk ? wbest[w] : gk;
           p = bkey_row(gk);
         }
-        const int pt = p % TG, pi = p / TG;
+        const int pt = p % TY, pi = p / TY;
         if (ty == pt) {   // the pivot row
 #pragma unroll
-          for (int i = 0; i < R; ++i)
+          for (int i = 0; i < RI; ++i)
             if (i == pi) {
 #pragma unroll
-              for (int j = 0; j < R; ++j) urow[tx + TG * j] = a[i][j];
+              for (int j = 0; j < RJ; ++j) urow[tx + TX * j] = a[i][j];
             }
           done |= 1u << pi;
         }
         if (tx == kq) {   // column k: the multipliers of every other row
 #pragma unroll
-          for (int i = 0; i < R; ++i) lcol[ty + TG * i] = (ty + TG * i == p) ? 0.0f : a[i][jj];
+          for (int i = 0; i < RI; ++i) lcol[ty + TY * i] = (ty + TY * i == p) ? 0.0f : a[i][jj];
         }
         if (t == 0) prow[k] = p;
         __syncthreads();
         const float pv = urow[k];
         if (t == 0 && !(pv != 0.0f && pv == pv)) s_bad = 1;
         const float rp = 1.0f / pv;
-        float fi[R], uj[R];
+        float fi[RI], uj[RJ];
 #pragma unroll
-        for (int i = 0; i < R; ++i) fi[i] = lcol[ty + TG * i];
+        for (int i = 0; i < RI; ++i) fi[i] = lcol[ty + TY * i];
 #pragma unroll
-        for (int j = 0; j < R; ++j) uj[j] = urow[tx + TG * j] * rp;
+        for (int j = 0; j < RJ; ++j) uj[j] = urow[tx + TX * j] * rp;
 #pragma unroll
-        for (int i = 0; i < R; ++i)
+        for (int i = 0; i < RI; ++i)
 #pragma unroll
-          for (int j = 0; j < R; ++j) a[i][j] -= fi[i] * uj[j];
+          for (int j = 0; j < RJ; ++j) a[i][j] -= fi[i] * uj[j];
         if (ty == pt) {
 #pragma unroll
-          for (int i = 0; i < R; ++i)
+          for (int i = 0; i < RI; ++i)
             if (i == pi) {
 #pragma unroll
-              for (int j = 0; j < R; ++j) a[i][j] = uj[j];
+              for (int j = 0; j < RJ; ++j) a[i][j] = uj[j];
             }
         }
         if (tx == kq) {
 #pragma unroll
-          for (int i = 0; i < R; ++i) a[i][jj] = (ty + TG * i == p) ? rp : -fi[i] * rp;
+          for (int i = 0; i < RI; ++i) a[i][jj] = (ty + TY * i == p) ? rp : -fi[i] * rp;
         }
       }
     }
@@ -200,21 +201,21 @@ __global__ void __launch_bounds__(TG * TG) batched_solve_kernel(BatchArgs g) {
       for (;;) {
         if (t < NMAX) cv[t] = t < n ? (float)rv[prow[t]] : 0.0f;
         __syncthreads();
-        float acc[R];
+        float acc[RI];
 #pragma unroll
-        for (int i = 0; i < R; ++i) {
+        for (int i = 0; i < RI; ++i) {
           float s = 0.0f;
 #pragma unroll
-          for (int j = 0; j < R; ++j) s = fmaf(a[i][j], cv[tx + TG * j], s);
+          for (int j = 0; j < RJ; ++j) s = fmaf(a[i][j], cv[tx + TX * j], s);
           acc[i] = s;
         }
 #pragma unroll
-        for (int off = TG / 2; off > 0; off >>= 1)
+        for (int off = TX / 2; off > 0; off >>= 1)
 #pragma unroll
-          for (int i = 0; i < R; ++i) acc[i] += __shfl_xor(acc[i], off);
+          for (int i = 0; i < RI; ++i) acc[i] += __shfl_xor(acc[i], off);
         if (tx == 0) {
 #pragma unroll
-          for (int i = 0; i < R; ++i) yv[ty + TG * i] = acc[i];
+          for (int i = 0; i < RI; ++i) yv[ty + TY * i] = acc[i];
         }
         __syncthreads();
         if (t < n) xv[t] += (double)yv[prow[t]];
@@ -259,20 +260,21 @@ __global__ void reset_counters_kernel(int* c, int k) {
 
 // ---- variants and buckets -------------------------------------------------------------
 struct Variant {
-  int tg, r;
+  int ty, tx, ri, rj;   // thread grid TY x TX, RI x RJ values per thread
   void (*kernel)(BatchArgs);
 };
 const Variant kVariants[] = {
-    {8, 4, batched_solve_kernel<8, 4>},    // n <= 32, one wave
-    {8, 8, batched_solve_kernel<8, 8>},    // n <= 64, one wave
-    {16, 4, batched_solve_kernel<16, 4>},  // n <= 64, four waves
-    {16, 8, batched_solve_kernel<16, 8>},  // n <= 128
-    {32, 4, batched_solve_kernel<32, 4>},  // n <= 128, sixteen waves
-    {32, 8, batched_solve_kernel<32, 8>},  // n <= 256
+    {8, 8, 4, 4, batched_solve_kernel<8, 8, 4, 4>},        // n <= 32, one wave
+    {8, 8, 8, 8, batched_solve_kernel<8, 8, 8, 8>},        // n <= 64, one wave
+    {16, 16, 4, 4, batched_solve_kernel<16, 16, 4, 4>},    // n <= 64, four waves
+    {16, 16, 8, 8, batched_solve_kernel<16, 16, 8, 8>},    // n <= 128
+    {32, 32, 4, 4, batched_solve_kernel<32, 32, 4, 4>},    // n <= 128, sixteen waves
+    {32, 32, 8, 8, batched_solve_kernel<32, 32, 8, 8>},    // n <= 256, sixteen waves (register spills)
+    {16, 32, 16, 8, batched_solve_kernel<16, 32, 16, 8>},  // n <= 256, eight waves
 };
 constexpr int kBuckets = 4;
 const int kBucketMax[kBuckets] = {32, 64, 128, 256};
-const std::vector<int> kCandidates[kBuckets] = {{0, 1, 2}, {1, 2}, {3, 4}, {5}};
+const std::vector<int> kCandidates[kBuckets] = {{0, 1, 2}, {1, 2}, {3, 4}, {6, 5}};
 
 }  // namespace
 
@@ -301,14 +303,14 @@ namespace {
 
 unsigned resident_grid(int v) {
   int per_cu = 0;
-  const int nt = kVariants[v].tg * kVariants[v].tg;
+  const int nt = kVariants[v].ty * kVariants[v].tx;
   ZH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kVariants[v].kernel),
                                                         nt, 0));
   return (unsigned)(std::max(per_cu, 1) * device_cu_count());
 }
 
 void launch_bucket(const BatchPlan& P, int b, int v, hipStream_t s) {
-  const int nt = kVariants[v].tg * kVariants[v].tg;
+  const int nt = kVariants[v].ty * kVariants[v].tx;
   const unsigned grid = std::min<unsigned>(P.grid[b], (unsigned)std::max(1, P.bucket_count[b]));
   BatchArgs args = P.args[b];
   void* kargs[] = {&args};
@@ -384,15 +386,26 @@ BatchPlan* batch_plan_create(int count, const int* n, const int* nrhs, const int
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
-  // one graph: reset the queues, then every bucket
-  hipStream_t cs;
-  ZH_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  ZH_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, cs, P->d_counters, kBuckets);
-  for (int bk = kBuckets - 1; bk >= 0; --bk)
-    if (P->bucket_count[bk] > 0) launch_bucket(*P, bk, P->variant[bk], cs);
-  ZH_CHECK(hipStreamEndCapture(cs, &P->graph));
-  ZH_CHECK(hipStreamDestroy(cs));
+  // one graph: reset the queues, then the buckets as independent branches (their
+  // workgroups only take work from their own queues, so the small buckets can use
+  // the CUs the large one leaves idle as its queue drains)
+  hipStream_t cs[kBuckets];
+  hipEvent_t ev[kBuckets + 1];
+  for (int i = 0; i < kBuckets; ++i) ZH_CHECK(hipStreamCreateWithFlags(&cs[i], hipStreamNonBlocking));
+  for (int i = 0; i <= kBuckets; ++i) ZH_CHECK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+  ZH_CHECK(hipStreamBeginCapture(cs[0], hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, cs[0], P->d_counters, kBuckets);
+  ZH_CHECK(hipEventRecord(ev[kBuckets], cs[0]));
+  for (int i = 1; i < kBuckets; ++i) ZH_CHECK(hipStreamWaitEvent(cs[i], ev[kBuckets], 0));
+  for (int bk = kBuckets - 1, i = 0; bk >= 0; --bk, ++i)   // the largest bucket on the capturing stream
+    if (P->bucket_count[bk] > 0) launch_bucket(*P, bk, P->variant[bk], cs[i]);
+  for (int i = 1; i < kBuckets; ++i) {
+    ZH_CHECK(hipEventRecord(ev[i], cs[i]));
+    ZH_CHECK(hipStreamWaitEvent(cs[0], ev[i], 0));
+  }
+  ZH_CHECK(hipStreamEndCapture(cs[0], &P->graph));
+  for (int i = 0; i < kBuckets; ++i) ZH_CHECK(hipStreamDestroy(cs[i]));
+  for (int i = 0; i <= kBuckets; ++i) ZH_CHECK(hipEventDestroy(ev[i]));
   ZH_CHECK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
   ZH_CHECK(hipStreamSynchronize(s));
   return P.release();
@@ -407,8 +420,8 @@ void batch_plan_status(BatchPlan* P, int* status, hipStream_t s) {
 
 void batch_plan_variants(const BatchPlan* P, int* tg, int* r, int* counts) {
   for (int bk = 0; bk < kBuckets; ++bk) {
-    tg[bk] = kVariants[P->variant[bk]].tg;
-    r[bk] = kVariants[P->variant[bk]].r;
+    tg[bk] = kVariants[P->variant[bk]].ty;
+    r[bk] = kVariants[P->variant[bk]].ri;
     counts[bk] = P->bucket_count[bk];
   }
 }

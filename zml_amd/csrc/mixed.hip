@@ -577,6 +577,8 @@ struct MixedCtx {
   std::vector<char> have_invu;
   float* inv_ready = nullptr; // L11^-1 the last diagonal-block panel already formed
   float* tmp = nullptr;      // inverse scratch
+  const float* l16_of = nullptr;   // L21 whose rounded copy l16 holds (one block column's updates share it)
+  int64_t l16_rows = 0, l16_kb = 0;
   // optimistic panels
   bool optimistic = true;
   float tau = 16.0f;         // largest |l| accepted below the diagonal block
@@ -598,7 +600,7 @@ void panel_gemm_f32(int64_t rows, int64_t cols, int64_t kb, const float* L, int6
 
 void trailing_gemm(int64_t rows, int64_t cols, int64_t kb, const float* L, int64_t ldl,
                    const float* U, int64_t ldu, float* C, int64_t ldc, hipStream_t s) {
-  const MixedCtx& mx = tl_mix;
+  MixedCtx& mx = tl_mix;
   if (mx.prec == kPrecF32) {
     void* tok = profile_begin(kProfGemmLowp, s);
     mp_gemm_f32(rows, cols, kb, -1.0f, L, ldl, U, ldu, 1.0f, nullptr, nullptr, C, ldc, s);
@@ -606,9 +608,16 @@ void trailing_gemm(int64_t rows, int64_t cols, int64_t kb, const float* L, int64
     return;
   }
   const int64_t kp = (kb + 7) / 8 * 8;
+  // the updates of one block column (next block, possibly in halves, then the
+  // rest) run in order on this stream with the same L21: round it once
+  const bool have_l = mx.l16_of == L && mx.l16_rows == rows && mx.l16_kb == kb;
+  mx.l16_of = L;
+  mx.l16_rows = rows;
+  mx.l16_kb = kb;
   if (mx.prec == kPrecF16) {
-    hipLaunchKernelGGL((rows_to_lowp_kernel<_Float16>), dim3(blocks(rows, 4)), dim3(256), 0, s, rows,
-                       (int)kb, L, ldl, mx.l16, kp, mx.rs);
+    if (!have_l)
+      hipLaunchKernelGGL((rows_to_lowp_kernel<_Float16>), dim3(blocks(rows, 4)), dim3(256), 0, s, rows,
+                         (int)kb, L, ldl, mx.l16, kp, mx.rs);
     hipLaunchKernelGGL((cols_to_lowp_t_kernel<_Float16>), dim3(blocks(cols, 32)), dim3(256), 0, s, cols,
                        (int)kb, U, ldu, mx.u16, kp, mx.cs);
     ZH_LAUNCH_CHECK();
@@ -618,8 +627,9 @@ void trailing_gemm(int64_t rows, int64_t cols, int64_t kb, const float* L, int64
   } else {
     __bf16* l = reinterpret_cast<__bf16*>(mx.l16);
     __bf16* u = reinterpret_cast<__bf16*>(mx.u16);
-    hipLaunchKernelGGL((rows_to_lowp_kernel<__bf16>), dim3(blocks(rows, 4)), dim3(256), 0, s, rows,
-                       (int)kb, L, ldl, l, kp, mx.rs);
+    if (!have_l)
+      hipLaunchKernelGGL((rows_to_lowp_kernel<__bf16>), dim3(blocks(rows, 4)), dim3(256), 0, s, rows,
+                         (int)kb, L, ldl, l, kp, mx.rs);
     hipLaunchKernelGGL((cols_to_lowp_t_kernel<__bf16>), dim3(blocks(cols, 32)), dim3(256), 0, s, cols,
                        (int)kb, U, ldu, u, kp, mx.cs);
     ZH_LAUNCH_CHECK();
@@ -1738,6 +1748,7 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   char* lp = static_cast<char*>(workspace(sizeof(_Float16) * 2 * (size_t)(n + NB_MIX) * kp +
                                               sizeof(float) * 2 * (n + NB_MIX) + 4096, 10));
   tl_mix.prec = opts.precision;
+  tl_mix.l16_of = nullptr;
   tl_mix.l16 = reinterpret_cast<_Float16*>(lp); lp += sizeof(_Float16) * (n + NB_MIX) * kp;
   tl_mix.u16 = reinterpret_cast<_Float16*>(lp); lp += sizeof(_Float16) * (n + NB_MIX) * kp;
   tl_mix.rs = reinterpret_cast<float*>(lp); lp += sizeof(float) * (n + NB_MIX);

@@ -19,7 +19,7 @@ struct Buf {
   size_t bytes = 0;
 };
 constexpr int kSlots = 24;
-constexpr int kAux = 4, kEvents = 8;
+constexpr int kAux = 4, kEvents = 12;
 constexpr int kSparing = 8;   // CU-sparing streams kept per device (distinct reservations)
 struct DeviceState {
   Buf slot[kSlots];
