@@ -10,7 +10,8 @@
 //   does, correctly rounded sqrt),
 //   TRSM the rows below (lower) / columns right of it (upper),
 //   update the rest of the block column only;
-// then SYRK the trailing triangle with K = 512 on the MFMA GEMM.
+// then SYRK the trailing triangle with K = 512 on the MFMA GEMM, the next
+// block column first so that its factorization overlaps the rest.
 // Row-major lower is column-major upper of the same memory (and vice versa),
 // which is how row-major calls are mapped.
 //
@@ -121,16 +122,16 @@ potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* _
 }  // namespace
 
 void potrf_f64_cm(int uplo, int64_t n, double* a, int64_t lda, int* info, hipStream_t s) {
+  if (n <= 0) return;
   const bool lower = uplo == kLower;
-  constexpr int64_t OB = 512;   // block column of the trailing SYRK (its K)
+  constexpr int64_t OB = 512;   // block column of the trailing updates (their K)
   auto at = [&](int64_t i, int64_t j) { return a + i + j * lda; };
-  for (int64_t j0 = 0; j0 < n; j0 += OB) {
-    const int64_t jb = std::min<int64_t>(OB, n - j0);
-    // factor block column [j0, j0 + jb) in DB-wide steps whose updates stay inside it
+  // factor block column [j0, j0 + jb) in DB-wide steps whose updates stay inside it
+  auto factor_block = [&](int64_t j0, int64_t jb, hipStream_t st) {
     for (int64_t k0 = j0; k0 < j0 + jb; k0 += DB) {
       const int64_t kb = std::min<int64_t>(DB, j0 + jb - k0);
       double* d = at(k0, k0);
-      hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, (int)kb, d, lda, lower ? 1 : 0, info, k0);
+      hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, (int)kb, d, lda, lower ? 1 : 0, info, k0);
       ZH_LAUNCH_CHECK();
       const int64_t r = n - k0 - kb;           // rows below (lower) / columns right (upper)
       if (r <= 0) break;
@@ -138,26 +139,58 @@ void potrf_f64_cm(int uplo, int64_t n, double* a, int64_t lda, int* info, hipStr
       if (lower) {
         // A21 <- A21 L11^-T ; the block column's part of A22 <- A22 - A21 A21^T
         double* a21 = at(k0 + kb, k0);
-        trsm_f64_cm(kRight, kLower, kTrans, kNonUnit, r, kb, 1.0, d, lda, a21, lda, s);
+        trsm_f64_cm(kRight, kLower, kTrans, kNonUnit, r, kb, 1.0, d, lda, a21, lda, st);
         if (w > 0)
-          gemm_f64_cm(kNoTrans, kTrans, r, w, kb, -1.0, a21, lda, a21, lda, 1.0, at(k0 + kb, k0 + kb), lda, s,
+          gemm_f64_cm(kNoTrans, kTrans, r, w, kb, -1.0, a21, lda, a21, lda, 1.0, at(k0 + kb, k0 + kb), lda, st,
                       kLower);
       } else {
         // A12 <- U11^-T A12 ; the block row's part of A22 <- A22 - A12^T A12
         double* a12 = at(k0, k0 + kb);
-        trsm_f64_cm(kLeft, kUpper, kTrans, kNonUnit, kb, r, 1.0, d, lda, a12, lda, s);
+        trsm_f64_cm(kLeft, kUpper, kTrans, kNonUnit, kb, r, 1.0, d, lda, a12, lda, st);
         if (w > 0)
-          gemm_f64_cm(kTrans, kNoTrans, w, r, kb, -1.0, a12, lda, a12, lda, 1.0, at(k0 + kb, k0 + kb), lda, s,
+          gemm_f64_cm(kTrans, kNoTrans, w, r, kb, -1.0, a12, lda, a12, lda, 1.0, at(k0 + kb, k0 + kb), lda, st,
                       kUpper);
       }
     }
+  };
+  if (n <= OB) {
+    factor_block(0, n, s);
+    return;
+  }
+  // Look-ahead of depth one: after block column j0 is factored, the caller's
+  // stream first updates the next block column and hands it to a panel stream,
+  // then applies the rest of the trailing update while the panel stream
+  // factors the next block column. Every update of a region stays ordered on
+  // the caller's stream; the panel stream touches only its own block column.
+  const hipStream_t sp = aux_stream(0);
+  const hipEvent_t ev_in = aux_event(0), ev_panel = aux_event(1), ev_next = aux_event(2);
+  ZH_CHECK(hipEventRecord(ev_in, s));
+  ZH_CHECK(hipStreamWaitEvent(sp, ev_in, 0));
+  factor_block(0, std::min<int64_t>(OB, n), sp);
+  ZH_CHECK(hipEventRecord(ev_panel, sp));
+  for (int64_t j0 = 0; j0 < n; j0 += OB) {
+    const int64_t jb = std::min<int64_t>(OB, n - j0);
     const int64_t r = n - j0 - jb;
     if (r <= 0) break;
-    if (lower)
-      syrk_f64_cm(kLower, kNoTrans, r, jb, -1.0, at(j0 + jb, j0), lda, 1.0, at(j0 + jb, j0 + jb), lda, s);
-    else
-      syrk_f64_cm(kUpper, kTrans, r, jb, -1.0, at(j0, j0 + jb), lda, 1.0, at(j0 + jb, j0 + jb), lda, s);
+    const int64_t jb2 = std::min<int64_t>(OB, r), r2 = r - jb2;
+    const int64_t c1 = j0 + jb, c2 = c1 + jb2;
+    ZH_CHECK(hipStreamWaitEvent(s, ev_panel, 0));
+    if (lower) {
+      const double* l21 = at(c1, j0);
+      gemm_f64_cm(kNoTrans, kTrans, r, jb2, jb, -1.0, l21, lda, l21, lda, 1.0, at(c1, c1), lda, s, kLower);
+      ZH_CHECK(hipEventRecord(ev_next, s));
+      if (r2 > 0) syrk_f64_cm(kLower, kNoTrans, r2, jb, -1.0, at(c2, j0), lda, 1.0, at(c2, c2), lda, s);
+    } else {
+      const double* u12 = at(j0, c1);
+      gemm_f64_cm(kTrans, kNoTrans, jb2, r, jb, -1.0, u12, lda, u12, lda, 1.0, at(c1, c1), lda, s, kUpper);
+      ZH_CHECK(hipEventRecord(ev_next, s));
+      if (r2 > 0) syrk_f64_cm(kUpper, kTrans, r2, jb, -1.0, at(j0, c2), lda, 1.0, at(c2, c2), lda, s);
+    }
+    ZH_CHECK(hipStreamWaitEvent(sp, ev_next, 0));
+    factor_block(c1, jb2, sp);
+    ZH_CHECK(hipEventRecord(ev_panel, sp));
   }
+  ZH_CHECK(hipStreamWaitEvent(s, ev_panel, 0));
 }
 
 }  // namespace zmlhip
