@@ -182,7 +182,8 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
     const int64_t jb = std::min(nb, mn - j0);
     const T* invb = ops.inv_store ? ops.inv_store(j0) : inv + (blk & 1) * nb * nb;
     ZH_CHECK(hipStreamWaitEvent(s, ev_panel, 0));
-    const int32_t* pr = compose_perm((int)j0, (int)(j0 + jb - 1), ipiv, s, 4);
+    const int32_t* pr = ops.block_perm ? ops.block_perm(j0) : nullptr;
+    if (!pr) pr = compose_perm((int)j0, (int)(j0 + jb - 1), ipiv, s, 4);
     const int64_t c1 = j0 + jb;                       // first column right of the block
     const int64_t jb2 = std::min(nb, std::max<int64_t>(0, mn - c1));   // next block width
     const int64_t rows = m - c1;
@@ -256,7 +257,7 @@ void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, i
     if (spare > device_cu_count() / 4) spare = 0;
   }
   const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64, nullptr, nullptr,
-                          nullptr, spare};
+                          nullptr, spare, nullptr};
   getrf_lookahead<double>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }
 

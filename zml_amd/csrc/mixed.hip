@@ -578,6 +578,8 @@ struct MixedCtx {
   float* inv_ready = nullptr; // L11^-1 the last diagonal-block panel already formed
   float* tmp = nullptr;      // inverse scratch
   const float* l16_of = nullptr;   // L21 whose rounded copy l16 holds (one block column's updates share it)
+  const int32_t* block_pairs[2] = {nullptr, nullptr};   // composed interchanges by block-column parity
+  int64_t block_pairs_j0[2] = {-1, -1};                 // ... of the block column starting here
   int64_t l16_rows = 0, l16_kb = 0;
   // optimistic panels
   bool optimistic = true;
@@ -838,10 +840,19 @@ void tri_inverse_f32(int64_t kb, const float* L, int64_t ld, float* inv, int uni
   }
 }
 
+// B (kb x cols) <- inv B. Only the update stream calls it (U12 of the next block
+// column and of the rest); a product of few output tiles (the next block
+// column's) splits K so that enough workgroups share it.
 void apply_inv_f32(int64_t kb, int64_t cols, const float* inv, float* B, int64_t ldb, float* tmp,
                    hipStream_t s) {
   ZH_CHECK(hipMemcpy2DAsync(tmp, sizeof(float) * cols, B, sizeof(float) * ldb, sizeof(float) * cols, kb,
                             hipMemcpyDeviceToDevice, s));
+  const int64_t tiles = ((kb + 127) / 128) * ((cols + 127) / 128);
+  if (tiles < 64 && kb >= 256) {
+    float* part = static_cast<float*>(workspace(sizeof(float) * mp_gemm_f32_splitk_floats(kb, cols, kb), 18));
+    mp_gemm_f32_splitk(kb, cols, kb, 1.0f, inv, kb, tmp, cols, 0.0f, B, ldb, part, s);
+    return;
+  }
   mp_gemm_f32(kb, cols, kb, 1.0f, inv, kb, tmp, cols, 0.0f, nullptr, nullptr, B, ldb, s);
 }
 
@@ -853,9 +864,12 @@ void apply_inv_f32(int64_t kb, int64_t cols, const float* inv, float* B, int64_t
 // linv (optional): the inverse of the block's unit lower factor is formed there
 // (its strictly upper part already zero) on a second stream, joined by the
 // event aux_event(5), while U11^-1 and L21 take this stream.
+// perm_k1 >= 0 (the block column's last diagonal block): its pivots complete
+// the block column's, whose interchanges are then composed on the second stream
+// too, into the buffer of the block column's parity (tl_mix.block_pairs).
 void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, int64_t src_ld, int32_t* ipiv,
                       int base, float* uinv, int64_t uinv_ld, hipStream_t s, float* linv = nullptr,
-                      int64_t linv_ld = 0) {
+                      int64_t linv_ld = 0, int perm_k1 = -1, const int32_t* perm_ipiv = nullptr) {
   MixedCtx& mx = tl_mix;
   hipLaunchKernelGGL(diag_getf2_kernel<DIAG_TY>, dim3(1), dim3(DIAG_TY * 32), 0, s, P, lda, ipiv, base, mx.pfail);
   ZH_LAUNCH_CHECK();
@@ -864,6 +878,10 @@ void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, i
     ZH_CHECK(hipEventRecord(aux_event(4), s));
     ZH_CHECK(hipStreamWaitEvent(s2, aux_event(4), 0));
     tri_inv256(P, lda, linv, linv_ld, 1, s2);
+    if (perm_k1 >= 0) {
+      const int par = (perm_k1 / NB_MIX) & 1;
+      mx.block_pairs[par] = compose_perm(perm_k1, perm_k1 + NB_MIX - 1, perm_ipiv, s2, 19 + par);
+    }
     ZH_CHECK(hipEventRecord(aux_event(5), s2));
   }
   // U11^-1 (upper) into scr1, whose strictly lower part must be zero
@@ -892,10 +910,17 @@ void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, i
 // accuracy, and GMRES-IR or the FP64 factorization take over if it cannot).
 // Otherwise the block column is restored and the caller factors it with the
 // full pivot search.
+const int32_t* block_perm_f32(int64_t j0) {
+  const MixedCtx& mx = tl_mix;
+  const int par = (int)((j0 / NB_MIX) & 1);
+  return mx.block_pairs_j0[par] == j0 ? mx.block_pairs[par] : nullptr;
+}
+
 bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int32_t* ipiv,
                    hipStream_t s) {
   MixedCtx& mx = tl_mix;
   const int64_t rows = m - j0;
+  mx.block_pairs_j0[(j0 / NB_MIX) & 1] = -1;
   if (!mx.optimistic || (jb != DB && jb != 2 * DB) || rows < jb) return false;
   ++mx.panels;
   mx.inv_ready = nullptr;
@@ -928,7 +953,7 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
       ZH_CHECK(hipMemcpy2DAsync(mx.ptmp, sizeof(float) * DB, A + (c2 + DB) * lda + c2, sizeof(float) * lda,
                                 sizeof(float) * DB, rows2 - DB, hipMemcpyDeviceToDevice, s));
     optimistic_block(A + c2 * lda + c2, lda, rows2, mx.ptmp, DB, ipiv + c2, (int)c2,
-                     ustore + DB * NB_MIX + DB, NB_MIX, s, linv + DB * jb + DB, jb);
+                     ustore + DB * NB_MIX + DB, NB_MIX, s, linv + DB * jb + DB, jb, (int)j0, ipiv);
     const int32_t* pr2 = compose_perm((int)c2, (int)(c2 + DB - 1), ipiv, s, 2);
     apply_perm<float>(DB, A + j0, lda, pr2, s);
     // the off-diagonal block -Lb^-1 Lba La^-1
@@ -946,6 +971,7 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     if (jb == NB_MIX) {
       mx.have_invu[j0 / NB_MIX] = 1;
       mx.inv_ready = mx.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;
+      mx.block_pairs_j0[(j0 / NB_MIX) & 1] = j0;
     }
     return true;
   }
@@ -1749,6 +1775,7 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
                                               sizeof(float) * 2 * (n + NB_MIX) + 4096, 10));
   tl_mix.prec = opts.precision;
   tl_mix.l16_of = nullptr;
+  tl_mix.block_pairs_j0[0] = tl_mix.block_pairs_j0[1] = -1;
   tl_mix.l16 = reinterpret_cast<_Float16*>(lp); lp += sizeof(_Float16) * (n + NB_MIX) * kp;
   tl_mix.u16 = reinterpret_cast<_Float16*>(lp); lp += sizeof(_Float16) * (n + NB_MIX) * kp;
   tl_mix.rs = reinterpret_cast<float*>(lp); lp += sizeof(float) * (n + NB_MIX);
@@ -1814,7 +1841,7 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
     static float* at(int64_t j0) { return tl_mix.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX; }
   };
   const LuOps<float> ops{trailing_gemm, trsm_f32, Inv::store, apply_inv_f32, panel_gemm_f32, try_panel_f32,
-                         InvAt::at, 8};
+                         InvAt::at, 8, block_perm_f32};
   getrf_lookahead<float>(n, n, S.W, n, S.ipiv, S.info, NB_MIX, ops, s);
   int info = 0;
   ZH_CHECK(hipMemcpyAsync(&info, S.info, sizeof(int), hipMemcpyDeviceToHost, s));

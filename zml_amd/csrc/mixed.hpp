@@ -16,6 +16,12 @@ void mp_gemm_bf16(int64_t m, int64_t n, int64_t K, float alpha, const __bf16* A,
 void mp_gemm_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, int64_t lda,
                  const float* B, int64_t ldb, float beta, const float* rs, const float* cs, float* C,
                  int64_t ldc, hipStream_t s, int lower = 0);
+// C = alpha A B + beta C (row-major FP32) for few output tiles and a long K:
+// K is split over up to 8 slices whose partial products (part: the floats
+// mp_gemm_f32_splitk_floats gives) are summed in order by a second kernel
+int64_t mp_gemm_f32_splitk_floats(int64_t m, int64_t n, int64_t K);
+void mp_gemm_f32_splitk(int64_t m, int64_t n, int64_t K, float alpha, const float* A, int64_t lda, const float* B,
+                        int64_t ldb, float beta, float* C, int64_t ldc, float* part, hipStream_t s);
 
 // Precision of the trailing update of the mixed factorizations.
 enum LowPrec : int { kPrecF32 = 0, kPrecF16 = 1, kPrecBF16 = 2 };
@@ -48,6 +54,9 @@ struct LuOps {
   // > 0: run the trailing updates on a stream that leaves this many CUs free,
   // so that the panel stream's workgroups always find empty CUs
   int spare_cus;
+  // optional: the composed interchanges of block column j0 when the panel
+  // already formed them (the driver composes them itself when null)
+  const int32_t* (*block_perm)(int64_t j0);
 };
 
 int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, int64_t lda,

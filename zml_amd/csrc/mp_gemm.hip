@@ -315,7 +315,19 @@ __global__ void __launch_bounds__(NTH, 2)
 mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __restrict__ A,
                  int64_t lda, const float* __restrict__ B, int64_t ldb, float beta,
                  const float* __restrict__ rs, const float* __restrict__ cs, float* __restrict__ C,
-                 int64_t ldc, int lower, int64_t tri_tiles) {
+                 int64_t ldc, int lower, int64_t tri_tiles, int64_t kslice, float* __restrict__ part) {
+  if (part) {   // split K: slice blockIdx.y of K, plain partial product into part[slice] (m x n)
+    const int64_t k0 = blockIdx.y * kslice;
+    A += k0;
+    B += k0 * ldb;
+    K = K - k0 < kslice ? K - k0 : kslice;
+    C = part + blockIdx.y * m * n;
+    ldc = n;
+    alpha = 1.0f;
+    beta = 0.0f;
+    rs = nullptr;
+    cs = nullptr;
+  }
   __shared__ float As[2][MT * LDAF];
   __shared__ float Bs[2][BKF * LDBF];
   const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
@@ -439,7 +451,40 @@ void mp_gemm_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, i
   const int64_t tri_tiles = lower ? tri_tile_count(kLower, ntm, ntn) : 0;
   const int64_t tiles = tri_tiles > 0 ? tri_tiles : ntm * ntn;
   hipLaunchKernelGGL(mp_gemm_f_kernel, dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha, A, lda,
-                     B, ldb, beta, rs, cs, C, ldc, lower, tri_tiles);
+                     B, ldb, beta, rs, cs, C, ldc, lower, tri_tiles, (int64_t)0, (float*)nullptr);
+  ZH_LAUNCH_CHECK();
+}
+
+namespace {
+// C = alpha sum_s P[s] + beta C (slices summed in order)
+__global__ void splitk_reduce_kernel(int64_t m, int64_t n, int slices, const float* __restrict__ P, float alpha,
+                                     float beta, float* __restrict__ C, int64_t ldc) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= m * n) return;
+  const int64_t i = e / n, j = e - i * n;
+  float acc = 0.0f;
+  for (int sl = 0; sl < slices; ++sl) acc += P[sl * m * n + e];
+  float* c = C + i * ldc + j;
+  *c = beta == 0.0f ? alpha * acc : beta * *c + alpha * acc;
+}
+}  // namespace
+
+int64_t mp_gemm_f32_splitk_floats(int64_t m, int64_t n, int64_t K) {
+  const int64_t slices = std::max<int64_t>(1, std::min<int64_t>(8, K / 64));
+  return slices * m * n;
+}
+
+void mp_gemm_f32_splitk(int64_t m, int64_t n, int64_t K, float alpha, const float* A, int64_t lda, const float* B,
+                        int64_t ldb, float beta, float* C, int64_t ldc, float* part, hipStream_t s) {
+  if (m <= 0 || n <= 0) return;
+  const int64_t slices = std::max<int64_t>(1, std::min<int64_t>(8, K / 64));
+  const int64_t kslice = (K + slices - 1) / slices;
+  const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
+  hipLaunchKernelGGL(mp_gemm_f_kernel, dim3((unsigned)(ntm * ntn), (unsigned)slices), dim3(NTH), 0, s, m, n, K,
+                     1.0f, A, lda, B, ldb, 0.0f, (const float*)nullptr, (const float*)nullptr, C, ldc, 0,
+                     (int64_t)0, kslice, part);
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256)), dim3(256), 0, s, m, n,
+                     (int)slices, (const float*)part, alpha, beta, C, ldc);
   ZH_LAUNCH_CHECK();
 }
 
