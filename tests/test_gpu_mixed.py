@@ -93,6 +93,26 @@ def test_mixed_lu_config3_backward_error(zh, torch_cuda, precision, order, delta
     assert res.panels == 1 and res.panels_optimistic == 1
 
 
+@pytest.mark.parametrize("precision,order,n", [(1, ROW, 2560), (2, COL, 2600), (1, COL, 2100)])
+def test_mixed_lu_several_block_columns_ahead(zh, torch_cuda, precision, order, n):
+    """Orders with four to six 512-wide block columns: every block column's
+    update of the next one, handed to the panel stream, and of the rest, with
+    the panels' interchanges composed on their second stream. Checked by the
+    FP64 backward error and the known solution."""
+    A = config3_matrix(n, 0.1, 11)
+    X0 = O.fill_uniform(ROW, n, 1, 45).reshape(n, 1)
+    B = A @ X0
+    X, res = run_mixed(zh, torch_cuda, A, B, order, precision)
+    be = backward_error(A, X, B)
+    cond_fe = np.abs(X - X0).max() / np.abs(X0).max()
+    print(f"n={n} prec={precision}: backward error {be:.3g}, IR {res.iterations}, GMRES {res.gmres_iterations}, "
+          f"fallback {res.fallback}, panels {res.panels_optimistic}/{res.panels}, error vs x0 {cond_fe:.3g}")
+    assert res.info == 0 and res.fallback == 0
+    assert be <= 16 and res.backward_error <= 16
+    assert res.iterations <= 5 and res.gmres_iterations == 0
+    assert cond_fe < 1e-8
+
+
 @pytest.mark.parametrize("precision", [0, 1, 2])
 def test_mixed_lu_well_conditioned_matches_oracle(zh, torch_cuda, precision):
     n, nrhs = 777, 3   # not a multiple of the block size
