@@ -43,40 +43,87 @@ __device__ __forceinline__ void tile_of(int64_t bid, int64_t ntm, int64_t ntn, i
 
 // C is read before the K loop (its HBM latency then overlaps the MFMA work:
 // for a rank-256 update the C traffic, not the arithmetic, bounds the kernel)
-// and combined with the accumulators at the end.
+// and combined with the accumulators at the end. Rows and columns beyond m, n
+// are clamped to the last valid ones, so every load is in bounds and none sits
+// behind a branch (the values read there are never stored).
 __device__ __forceinline__ void load_c(f16v (&cv)[2][2], int64_t m, int64_t n, float beta,
                                        const float* __restrict__ C, int64_t ldc, int64_t r0, int64_t c0,
                                        int lane) {
+  if (beta == 0.0f) {
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) cv[bi][bj][r] = 0.0f;
+    return;
+  }
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj) {
-      const int64_t j = c0 + 32 * bj + (lane & 31);
+      const int64_t j = min(c0 + 32 * bj + (lane & 31), n - 1);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int64_t i = r0 + 32 * bi + 8 * (r >> 2) + 4 * ((lane >> 5) & 1) + (r & 3);
-        cv[bi][bj][r] = (beta != 0.0f && i < m && j < n) ? C[i * ldc + j] : 0.0f;
+        const int64_t i = min(r0 + 32 * bi + 8 * (r >> 2) + 4 * ((lane >> 5) & 1) + (r & 3), m - 1);
+        cv[bi][bj][r] = C[i * ldc + j];
       }
     }
 }
 
+// C = beta C + alpha rs_i cs_j acc on a wave's 64 x 64 block. The scales are
+// all requested before the first store (clamped like the C loads), and a block
+// that lies inside C is stored without per-element tests.
 __device__ __forceinline__ void epilogue(const f16v (&acc)[2][2], const f16v (&cv)[2][2], int64_t m,
                                          int64_t n, float alpha, float beta, const float* __restrict__ rs,
                                          const float* __restrict__ cs, float* __restrict__ C,
                                          int64_t ldc, int64_t r0, int64_t c0, int lane) {
+  float sc[2], sr[2][16];
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) sc[bj] = alpha;
+  if (cs) {
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) sc[bj] = alpha * cs[min(c0 + 32 * bj + (lane & 31), n - 1)];
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) sr[bi][r] = 1.0f;
+  if (rs) {
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        sr[bi][r] = rs[min(r0 + 32 * bi + 8 * (r >> 2) + 4 * ((lane >> 5) & 1) + (r & 3), m - 1)];
+  }
+  auto value = [&](int bi, int bj, int r) {
+    const float s = sc[bj] * sr[bi][r];
+    return beta == 0.0f ? s * acc[bi][bj][r] : beta * cv[bi][bj][r] + s * acc[bi][bj][r];
+  };
+  if (r0 + 64 <= m && c0 + 64 <= n) {
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const int64_t j = c0 + 32 * bj + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t i = r0 + 32 * bi + 8 * (r >> 2) + 4 * ((lane >> 5) & 1) + (r & 3);
+          C[i * ldc + j] = value(bi, bj, r);
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj) {
       const int64_t j = c0 + 32 * bj + (lane & 31);
       if (j >= n) continue;
-      const float cj = cs ? cs[j] : 1.0f;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int64_t i = r0 + 32 * bi + 8 * (r >> 2) + 4 * ((lane >> 5) & 1) + (r & 3);
-        if (i >= m) continue;
-        const float s = alpha * cj * (rs ? rs[i] : 1.0f);
-        C[i * ldc + j] = beta == 0.0f ? s * acc[bi][bj][r] : beta * cv[bi][bj][r] + s * acc[bi][bj][r];
+        if (i < m) C[i * ldc + j] = value(bi, bj, r);
       }
     }
 }
