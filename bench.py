@@ -448,8 +448,8 @@ def bench_batched(args):
                 "frac": round(bytes_ / t / 8e12, 4)},
         "refinement_steps_max": int(status.max()), "not_converged": int((status < 0).sum()),
         "backward_error_sample_max": round(worst, 4),
-        "bucket_shapes": [{"n_max": (32, 64, 128, 256)[b], "systems": cnt[b], "threads": tg[b] * tg[b],
-                           "values_per_thread": rr[b] * rr[b]} for b in range(4)],
+        "bucket_shapes": [{"n_max": (32, 64, 128, 256)[b], "systems": cnt[b], "thread_rows": tg[b],
+                           "matrix_rows_per_thread": rr[b]} for b in range(4)],
         "plan_seconds": round(t_plan, 3),
     }
     if not args.no_cpu_baseline:

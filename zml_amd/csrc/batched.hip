@@ -77,12 +77,14 @@ __device__ __forceinline__ double block_max(double v, double* red, int t) {
 
 template <int TY, int TX, int RI, int RJ>
 __global__ void __launch_bounds__(TY * TX) batched_solve_kernel(BatchArgs g) {
-  constexpr int NT = TY * TX, NMAX = TY * RI, NW = (NT + 63) / 64;
+  constexpr int NT = TY * TX, NMAX = TY * RI, NW = (NT + 63) / 64, PARTS = NT / NMAX;
   static_assert(TX * RJ == NMAX && TX <= 64 && RI <= 32, "thread grid shape");
+  static_assert(PARTS >= 1 && PARTS * NMAX == NT, "residual split");
   __shared__ uint64_t wbest[NW];
   __shared__ float lcol[NMAX], urow[NMAX], cv[NMAX], yv[NMAX];
   __shared__ int prow[NMAX];
   __shared__ double xv[NMAX], rv[NMAX], red[NW];
+  __shared__ double psum[NT], pabs[NT];   // partial residual sums and |a| row sums
   __shared__ int s_idx, s_bad;
   const int t = threadIdx.x, tx = t % TX, ty = t / TX, lane = t & 63, wv = t >> 6;
   for (;;) {
@@ -96,14 +98,23 @@ __global__ void __launch_bounds__(TY * TX) batched_solve_kernel(BatchArgs g) {
     const int n = d.n;
 
     // ---- FP32 Gauss-Jordan inversion, in registers
+    // a thread row that lies inside n x n is loaded without masks (its RJ
+    // loads in flight together); rows at the edge test every element
     float a[RI][RJ];
 #pragma unroll
-    for (int i = 0; i < RI; ++i)
+    for (int i = 0; i < RI; ++i) {
+      const int row = ty + TY * i;
+      if (row < n && tx + TX * (RJ - 1) < n) {
 #pragma unroll
-      for (int j = 0; j < RJ; ++j) {
-        const int row = ty + TY * i, col = tx + TX * j;
-        a[i][j] = (row < n && col < n) ? (float)d.a[row + (int64_t)col * n] : 0.0f;
+        for (int j = 0; j < RJ; ++j) a[i][j] = (float)d.a[row + (int64_t)(tx + TX * j) * n];
+      } else {
+#pragma unroll
+        for (int j = 0; j < RJ; ++j) {
+          const int col = tx + TX * j;
+          a[i][j] = (row < n && col < n) ? (float)d.a[row + (int64_t)col * n] : 0.0f;
+        }
       }
+    }
     unsigned done = 0;
 #pragma unroll
     for (int i = 0; i < RI; ++i)
@@ -220,14 +231,42 @@ __global__ void __launch_bounds__(TY * TX) batched_solve_kernel(BatchArgs g) {
         __syncthreads();
         if (t < n) xv[t] += (double)yv[prow[t]];
         __syncthreads();
-        // r = b - A x, ||A||_inf on the first pass
+        // r = b - A x, ||A||_inf on the first pass. Thread t takes row t % NMAX
+        // and the columns t / NMAX + PARTS l, eight loads in flight at a time (a
+        // tail of fewer than eight one by one); the PARTS partial sums of a row
+        // are then added in a fixed order.
+        {
+          const int row = t % NMAX, part = t / NMAX;
+          double accd = 0.0, asum = 0.0;
+          if (row < n) {
+            int j0 = part;
+            for (; j0 + 7 * PARTS < n; j0 += 8 * PARTS) {   // eight whole columns: no masks
+              double v[8];
+#pragma unroll
+              for (int u = 0; u < 8; ++u) v[u] = d.a[row + (int64_t)(j0 + u * PARTS) * n];
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                accd -= v[u] * xv[j0 + u * PARTS];
+                asum += fabs(v[u]);
+              }
+            }
+            for (; j0 < n; j0 += PARTS) {
+              const double v = d.a[row + (int64_t)j0 * n];
+              accd -= v * xv[j0];
+              asum += fabs(v);
+            }
+          }
+          psum[t] = accd;
+          pabs[t] = asum;
+        }
+        __syncthreads();
         double ri = 0.0, ai = 0.0;
         if (t < n) {
           double accd = b[t], asum = 0.0;
-          for (int j = 0; j < n; ++j) {
-            const double v = d.a[t + (int64_t)j * n];
-            accd -= v * xv[j];
-            asum += fabs(v);
+#pragma unroll
+          for (int q = 0; q < PARTS; ++q) {
+            accd += psum[q * NMAX + t];
+            asum += pabs[q * NMAX + t];
           }
           rv[t] = accd;
           ri = fabs(accd);

@@ -191,7 +191,10 @@ trsm_lower_f32_kernel(int kb, const float* __restrict__ L, int64_t ldl, int unit
   if (col >= cols) return;
   float x[64];
 #pragma unroll
-  for (int r = 0; r < 64; ++r) x[r] = r < kb ? B[r * ldb + col] : 0.0f;
+  for (int r = 0; r < 64; ++r) {   // rows beyond kb: a valid row is read and masked (no branch per load)
+    const float v = B[(r < kb ? r : kb - 1) * ldb + col];
+    x[r] = r < kb ? v : 0.0f;
+  }
 #pragma unroll
   for (int k = 0; k < 64; ++k) {
     if (k < kb) {

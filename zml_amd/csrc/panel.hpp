@@ -320,9 +320,13 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
     ak = pick<T>(a, k + 1);   // candidate value of the next column
   }
 
+  // nothing is in flight any more: the row's stores go out back to back
+  __builtin_amdgcn_s_waitcnt(0xF70);
+  if (own) {
 #pragma unroll
-  for (int c = 0; c < W; ++c)
-    if (own && c < Wn) P[i * ld + c] = a[c];
+    for (int c = 0; c < W; ++c)
+      if (c < Wn) P[i * ld + c] = a[c];
+  }
   if (g == 0 && t < Wn) ipiv[t] = s_piv[t];
   if (g == 0 && t == 0 && first_zero != 0 && *info == 0) *info = (int)(info_base + first_zero);
 }
@@ -394,27 +398,36 @@ __global__ void compose_perm_kernel(int k1, int k2, const int32_t* __restrict__ 
 }
 
 // Applies the moves to columns [0, ncols) of a row-major matrix: every
-// workgroup owns 16 columns, reads all source rows, then writes them.
+// workgroup owns 16 columns, copies the move list into LDS, reads all source
+// rows (no load waits on another: the row numbers come from LDS), then writes
+// them.
 template <typename T>
 __global__ void __launch_bounds__(256)
 apply_perm_kernel(int64_t ncols, T* __restrict__ a, int64_t lda, const int32_t* __restrict__ pairs) {
-  constexpr int CW = 16, RPT = PERM_MAX / (256 / CW);   // 64 moves per thread
+  constexpr int CW = 16, RG = 256 / CW, RPT = PERM_MAX / RG;   // 64 moves per thread
+  __shared__ int32_t mv[2 * PERM_MAX];                          // (destination, source) per move
   const int n = pairs[0];
+  for (int q = threadIdx.x; q < 2 * n; q += 256) mv[q] = pairs[1 + q];
+  __syncthreads();
   const int col = threadIdx.x % CW, q0 = threadIdx.x / CW;
   const int64_t c = blockIdx.x * (int64_t)CW + col;
-  const bool in = c < ncols;
+  const int64_t cl = c < ncols ? c : ncols - 1;   // loads stay in bounds; only c < ncols is written
   T v[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    const int q = q0 + r * (256 / CW);
-    if (in && q < n) v[r] = a[(int64_t)pairs[2 + 2 * q] * lda + c];
+    if (r * RG >= n) break;
+    const int q = q0 + r * RG;
+    v[r] = a[(int64_t)mv[2 * (q < n ? q : 0) + 1] * lda + cl];
   }
-  // a column chunk belongs to one workgroup: all of its reads precede all of its writes
+  // a column chunk belongs to one workgroup: all of its reads (every load
+  // returned: s_waitcnt vmcnt(0)) precede all of its writes
+  __builtin_amdgcn_s_waitcnt(0xF70);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    const int q = q0 + r * (256 / CW);
-    if (in && q < n) a[(int64_t)pairs[1 + 2 * q] * lda + c] = v[r];
+    if (r * RG >= n) break;
+    const int q = q0 + r * RG;
+    if (q < n && c < ncols) a[(int64_t)mv[2 * q] * lda + c] = v[r];
   }
 }
 

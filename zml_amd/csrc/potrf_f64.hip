@@ -43,12 +43,24 @@ potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* _
   if (*info != 0) return;   // an earlier block already failed
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   double v[8][8];
+  // every address of the nb x nb block is readable: all 36 loads are issued
+  // unconditionally (clamped indices; the empty asm keeps the compiler from
+  // sinking each one under its mask and waiting for it), then masked
 #pragma unroll
   for (int ii = 0; ii < 8; ++ii)
 #pragma unroll
     for (int cc = 0; cc <= ii; ++cc) {
       const int r = tx + 16 * ii, c = ty + 16 * cc;
-      v[ii][cc] = (r < nb && c <= r) ? (lower ? a[r + (int64_t)c * lda] : a[c + (int64_t)r * lda]) : 0.0;
+      const int64_t rc = r < nb ? r : nb - 1, cl = c < nb ? c : nb - 1;
+      v[ii][cc] = a[lower ? rc + cl * lda : cl + rc * lda];
+    }
+#pragma unroll
+  for (int ii = 0; ii < 8; ++ii)
+#pragma unroll
+    for (int cc = 0; cc <= ii; ++cc) {
+      asm volatile("" : "+v"(v[ii][cc]));
+      const int r = tx + 16 * ii, c = ty + 16 * cc;
+      if (!(r < nb && c <= r)) v[ii][cc] = 0.0;
     }
   // column 0 (ty == 0, cc == 0)
   if (ty == 0) {
@@ -111,10 +123,7 @@ potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* _
 #pragma unroll
     for (int cc = 0; cc <= ii; ++cc) {
       const int r = tx + 16 * ii, c = ty + 16 * cc;
-      if (r < nb && c <= r && c < done) {
-        if (lower) a[r + (int64_t)c * lda] = v[ii][cc];
-        else a[c + (int64_t)r * lda] = v[ii][cc];
-      }
+      if (r < nb && c <= r && c < done) a[lower ? r + (int64_t)c * lda : c + (int64_t)r * lda] = v[ii][cc];
     }
   if (fail && t == 0) *info = (int)(info_base + fail);
 }
