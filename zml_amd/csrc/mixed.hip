@@ -283,9 +283,15 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
       best = other > best ? other : best;
       if (lane == kq) wbest[wv] = best;
       __syncthreads();
-      uint64_t g = wbest[0];
+      // the workgroup's pivot: lanes 0 .. NWV - 1 of every wave reduce the wave keys
+      uint64_t g = lane < NWV ? wbest[lane] : 0;
 #pragma unroll
-      for (int w = 1; w < NWV; ++w) g = wbest[w] > g ? wbest[w] : g;
+      for (int off = 1; off < NWV; off <<= 1) {
+        const uint64_t o = __shfl_xor(g, off);
+        g = o > g ? o : g;
+      }
+      g = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(g >> 32), 0) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((uint32_t)g, 0);
       const int p = key_row(g), pt = p % TY, pi = p / TY;
       const float pv = colk[p];
       const bool ok = pv != 0.0f && pv == pv;
@@ -305,11 +311,20 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
       // every thread forms its own rows' multipliers from the published column
       const bool recip = fabsf(pv) >= 0x1p-126f;
       const float rp = 1.0f / pv;
+      // (recip is the same in every thread: the division runs only for a subnormal pivot)
       float li[RI];
+      if (recip) {
 #pragma unroll
-      for (int i = 0; i < RI; ++i) {
-        const float c = colk[ty + TY * i];
-        li[i] = (!((done >> i) & 1u) && ok) ? (recip ? c * rp : c / pv) : 0.0f;
+        for (int i = 0; i < RI; ++i) {
+          const float c = colk[ty + TY * i];
+          li[i] = (!((done >> i) & 1u) && ok) ? c * rp : 0.0f;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < RI; ++i) {
+          const float c = colk[ty + TY * i];
+          li[i] = (!((done >> i) & 1u) && ok) ? c / pv : 0.0f;
+        }
       }
       __syncthreads();
       if (tx == kq) {
@@ -354,6 +369,7 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
     for (int j = 0; j < RJ; ++j) P[row * ld + tx + TX * j] = a[i][j];
   }
 }
+
 
 // *fail |= 2 when some |x| of the rows x cols block is not <= tau (NaN included)
 __global__ void bound_check_kernel(int64_t rows, int cols, const float* __restrict__ X, int64_t ld,
