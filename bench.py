@@ -52,7 +52,7 @@ PREC = {"fp32": 0, "fp16": 1, "bf16": 2}
 PROF_GEMM_F64, PROF_GEMM_LOWP = 1, 2
 
 
-def cpu_baseline(n=2048, reps=2):
+def cpu_baseline(n=4096, reps=1):
     """zml's getrf + getrs (oracle/zml_oracle.c restatement), one core."""
     import numpy as np
     import oracle as O
@@ -457,7 +457,7 @@ def bench_batched(args):
     print(json.dumps(out))
 
 
-def cpu_baseline_batched(hA, hB, ns, nr, spd, oa, ob, sample=4000):
+def cpu_baseline_batched(hA, hB, ns, nr, spd, oa, ob, sample=20000):
     """zml's gesv (general) and posv (SPD) restated in C, one system per
     thread on every host core, over the first `sample` systems."""
     import concurrent.futures as cf
@@ -480,7 +480,7 @@ def cpu_baseline_batched(hA, hB, ns, nr, spd, oa, ob, sample=4000):
         list(ex.map(solve, idx))
     t = time.perf_counter() - t0
     return {"value": round(len(idx) / t, 1), "unit": "systems/s", "cores": cores, "kind": "port",
-            "sample": f"first {len(idx)} systems of the same batch, zml gesv/posv restated in C "
+            "sample": f"{'all' if len(idx) == len(ns) else 'first'} {len(idx)} systems of the same batch, zml gesv/posv restated in C "
                       f"(oracle/zml_oracle.c), one system per thread, {t:.2f} s"}
 
 

@@ -123,7 +123,7 @@ __device__ __forceinline__ T pick(const T (&a)[W], int c) {
 
 // The empty asm statements keep the compiler from hoisting every group's
 // pivot-row reads (LDS) ahead of the first group: only 8 values of u are live
-// at a time, which keeps the FP64 leaf within two waves per SIMD.
+// at a time, which keeps the FP64 leaf free of register spills.
 #define ZH_GROUP_FENCE() asm volatile("" ::: "memory")
 template <typename T>
 __device__ __forceinline__ void update_row(T (&a)[W], T l, const T* __restrict__ u, int k) {
