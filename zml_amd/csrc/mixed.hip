@@ -875,6 +875,21 @@ void apply_inv_f32(int64_t kb, int64_t cols, const float* inv, float* B, int64_t
   mp_gemm_f32(kb, cols, kb, 1.0f, inv, kb, tmp, cols, 0.0f, nullptr, nullptr, B, ldb, s);
 }
 
+// C = alpha A B + beta C (FP32, row-major) for the optimistic panels: a
+// product of few output tiles (the 256 x 256 blocks, L21 of a short panel)
+// splits K so that enough workgroups share it. Its partial sums use their own
+// workspace slot, apart from the update stream's.
+void panel_product_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, int64_t lda, const float* B,
+                       int64_t ldb, float beta, float* C, int64_t ldc, hipStream_t s) {
+  const int64_t tiles = ((m + 127) / 128) * ((n + 127) / 128);
+  if (tiles < 128 && K >= 128) {
+    float* part = static_cast<float*>(workspace(sizeof(float) * mp_gemm_f32_splitk_floats(m, n, K), 21));
+    mp_gemm_f32_splitk(m, n, K, alpha, A, lda, B, ldb, beta, C, ldc, part, s);
+    return;
+  }
+  mp_gemm_f32(m, n, K, alpha, A, lda, B, ldb, beta, nullptr, nullptr, C, ldc, s);
+}
+
 // One optimistic 256-wide block column: partial pivoting inside its diagonal
 // block P (ld lda, rows x DB with rows >= DB), then L21 = A21 U11^-1 with A21
 // taken from src21 (rows - DB x DB, ld src_ld; P's own rows below the block
@@ -911,7 +926,7 @@ void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, i
   ZH_CHECK(hipMemcpy2DAsync(uinv, sizeof(float) * uinv_ld, mx.scr1, sizeof(float) * DB, sizeof(float) * DB, DB,
                             hipMemcpyDeviceToDevice, s));
   if (rows > DB) {
-    mp_gemm_f32(rows - DB, DB, DB, 1.0f, src21, src_ld, mx.scr1, DB, 0.0f, nullptr, nullptr, P + DB * lda, lda, s);
+    panel_product_f32(rows - DB, DB, DB, 1.0f, src21, src_ld, mx.scr1, DB, 0.0f, P + DB * lda, lda, s);
     hipLaunchKernelGGL(bound_check_kernel, dim3(blocks((rows - DB) * DB / 16)), dim3(256), 0, s, rows - DB, DB,
                        (const float*)(P + DB * lda), lda, mx.tau, mx.pfail);
     ZH_LAUNCH_CHECK();
@@ -965,9 +980,9 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     // U12 of the first half = La^-1 A12, then A22 -= L21 U12 within the block column
     ZH_CHECK(hipMemcpy2DAsync(mx.scr4, sizeof(float) * DB, A + j0 * lda + c2, sizeof(float) * lda,
                               sizeof(float) * DB, DB, hipMemcpyDeviceToDevice, s));
-    mp_gemm_f32(DB, DB, DB, 1.0f, linv, jb, mx.scr4, DB, 0.0f, nullptr, nullptr, A + j0 * lda + c2, lda, s);
-    mp_gemm_f32(rows2, DB, DB, -1.0f, A + c2 * lda + j0, lda, A + j0 * lda + c2, lda, 1.0f, nullptr, nullptr,
-                A + c2 * lda + c2, lda, s);
+    panel_product_f32(DB, DB, DB, 1.0f, linv, jb, mx.scr4, DB, 0.0f, A + j0 * lda + c2, lda, s);
+    panel_product_f32(rows2, DB, DB, -1.0f, A + c2 * lda + j0, lda, A + j0 * lda + c2, lda, 1.0f, A + c2 * lda + c2,
+                      lda, s);
     if (rows2 > DB)
       ZH_CHECK(hipMemcpy2DAsync(mx.ptmp, sizeof(float) * DB, A + (c2 + DB) * lda + c2, sizeof(float) * lda,
                                 sizeof(float) * DB, rows2 - DB, hipMemcpyDeviceToDevice, s));
@@ -977,9 +992,8 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     apply_perm<float>(DB, A + j0, lda, pr2, s);
     // the off-diagonal block -Lb^-1 Lba La^-1
     ZH_CHECK(hipStreamWaitEvent(s, aux_event(5), 0));   // Lb^-1
-    mp_gemm_f32(DB, DB, DB, 1.0f, A + c2 * lda + j0, lda, linv, jb, 0.0f, nullptr, nullptr, mx.scr3, DB, s);
-    mp_gemm_f32(DB, DB, DB, -1.0f, linv + DB * jb + DB, jb, mx.scr3, DB, 0.0f, nullptr, nullptr, linv + DB * jb,
-                jb, s);
+    panel_product_f32(DB, DB, DB, 1.0f, A + c2 * lda + j0, lda, linv, jb, 0.0f, mx.scr3, DB, s);
+    panel_product_f32(DB, DB, DB, -1.0f, linv + DB * jb + DB, jb, mx.scr3, DB, 0.0f, linv + DB * jb, jb, s);
   }
   int fail = 0;
   ZH_CHECK(hipMemcpyAsync(&fail, mx.pfail, sizeof(int), hipMemcpyDeviceToHost, s));
