@@ -879,10 +879,11 @@ void apply_inv_f32(int64_t kb, int64_t cols, const float* inv, float* B, int64_t
 // product of few output tiles (the 256 x 256 blocks, L21 of a short panel)
 // splits K so that enough workgroups share it. Its partial sums use their own
 // workspace slot, apart from the update stream's.
+constexpr size_t kPanelSplitBytes = sizeof(float) * 8 * 128 * 128 * 128;   // 8 slices x 128 tiles
 void panel_product_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, int64_t lda, const float* B,
                        int64_t ldb, float beta, float* C, int64_t ldc, hipStream_t s) {
   const int64_t tiles = ((m + 127) / 128) * ((n + 127) / 128);
-  if (tiles < 128 && K >= 128) {
+  if (tiles < 128 && K >= 128) {   // (the solvers reserve kPanelSplitBytes up front)
     float* part = static_cast<float*>(workspace(sizeof(float) * mp_gemm_f32_splitk_floats(m, n, K), 21));
     mp_gemm_f32_splitk(m, n, K, alpha, A, lda, B, ldb, beta, C, ldc, part, s);
     return;
@@ -1642,7 +1643,7 @@ void chol_factor(Solver& S, const CholWork& cw, int prec, double shift) {
       float* A21 = S.W + (j0 + jb) * n + j0;
       ZH_CHECK(hipMemcpy2DAsync(cw.tmp, sizeof(float) * jb, A21, sizeof(float) * n, sizeof(float) * jb, m2,
                                 hipMemcpyDeviceToDevice, sp));
-      mp_gemm_f32(m2, jb, jb, 1.0f, cw.tmp, jb, invLT, jb, 0.0f, nullptr, nullptr, A21, n, sp);
+      panel_product_f32(m2, jb, jb, 1.0f, cw.tmp, jb, invLT, jb, 0.0f, A21, n, sp);
       if (prec == kPrecF32) {
         hipLaunchKernelGGL(transpose_rect_f32_kernel, dim3(blocks(jb, 32), blocks(m2, 32)), dim3(32, 8), 0, sp, m2,
                            jb, (const float*)A21, n, cw.l21t[par], m2);
@@ -1777,6 +1778,7 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   S.precond = lu_solve;
   // ---- workspaces (slots 8..): the FP32 factors dominate (4 n^2 bytes)
   S.W = static_cast<float*>(workspace(sizeof(float) * (size_t)n * n, 8));
+  workspace(kPanelSplitBytes, 21);   // split-K partial sums of the panel products
   char* small = static_cast<char*>(workspace(
       sizeof(float) * 2 * (size_t)S.nblk * NB_MIX * NB_MIX + sizeof(int32_t) * 2 * n + 64 +
           sizeof(double) * 3 * n + sizeof(float) * (n + NB_MIX) + sizeof(uint64_t) * (n + NB_MIX) + 64 +
@@ -2026,6 +2028,7 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
   S.A = S64;
   S.lda = n;
   S.W = static_cast<float*>(workspace(sizeof(float) * (size_t)n * n, 8));
+  workspace(kPanelSplitBytes, 21);   // split-K partial sums of the panel products
   const size_t blk2 = (size_t)NB_MIX * NB_MIX;
   const SkinnySplit ssp = skinny_split(n);
   char* small = static_cast<char*>(workspace(
