@@ -55,16 +55,28 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
 // A "line" is a row of a row-major matrix or a column of a column-major one
 // (its elements are contiguous); a "position" indexes along a line.
 // out[line] = max_k |a[line ld + k]| * (scale ? scale[k] : 1); one wave per line.
+// (sum: also the line's sum of |a|, in the same pass)
 __global__ void line_absmax_kernel(int64_t n, const double* __restrict__ a, int64_t lda,
-                                   const double* __restrict__ scale, double* __restrict__ out) {
+                                   const double* __restrict__ scale, double* __restrict__ out,
+                                   double* __restrict__ sum = nullptr) {
   const int64_t line = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (line >= n) return;
   const double* p = a + line * lda;
-  double acc = 0.0;
-  for (int64_t k = lane; k < n; k += 64) acc = fmax(acc, scale ? fabs(p[k]) * scale[k] : fabs(p[k]));
-  for (int off = 32; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off));
-  if (lane == 0) out[line] = acc;
+  double acc = 0.0, tot = 0.0;
+  for (int64_t k = lane; k < n; k += 64) {
+    const double v = fabs(p[k]);
+    acc = fmax(acc, scale ? v * scale[k] : v);
+    tot += v;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    acc = fmax(acc, __shfl_xor(acc, off));
+    tot += __shfl_xor(tot, off);
+  }
+  if (lane == 0) {
+    out[line] = acc;
+    if (sum) sum[line] = tot;
+  }
 }
 
 // out[k] = max over lines of |a[line ld + k]| * (scale ? scale[line] : 1): one
@@ -1822,12 +1834,16 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   double* rmax = S.r;
   double* cmax = S.c;
   const int64_t split = std::max<int64_t>(1, std::min<int64_t>(256, n / 64)), chunk = (n + split - 1) / split;
-  if (layout == kRowMajor)
+  if (layout == kRowMajor) {
+    // row maxima and ||A||_inf of the original matrix in one pass
     hipLaunchKernelGGL(line_absmax_kernel, dim3(blocks(n, 4)), dim3(256), 0, s, n, A, lda,
-                       (const double*)nullptr, rmax);
-  else
+                       (const double*)nullptr, rmax, S.rowsum);
+    ZH_LAUNCH_CHECK();
+    S.anorm = reduce_host(S, n, S.rowsum, nullptr, 1);
+  } else {
     hipLaunchKernelGGL(cross_absmax_kernel, dim3(blocks(n), (unsigned)split), dim3(256), 0, s, n, A, lda,
                        (const double*)nullptr, rmax, chunk);
+  }
   hipLaunchKernelGGL(to_scale_kernel, dim3(blocks(n)), dim3(256), 0, s, n, rmax, S.rowsum);   // rowsum <- r
   if (layout == kRowMajor)
     hipLaunchKernelGGL(cross_absmax_kernel, dim3(blocks(n), (unsigned)split), dim3(256), 0, s, n, A, lda,
@@ -1841,11 +1857,11 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   hipLaunchKernelGGL(equilibrate_kernel, dim3(blocks(n, 32), blocks(n, 32)), dim3(32, 8), 0, s, layout, n,
                      A, lda, S.r, S.c, S.W);
   ZH_LAUNCH_CHECK();
-  // ||A||_inf of the original matrix
-  hipLaunchKernelGGL(rowsum_abs_kernel, dim3(layout == kColMajor ? blocks(n) : (unsigned)n), dim3(256), 0, s,
-                     layout, n, A, lda, S.rowsum);
-  ZH_LAUNCH_CHECK();
-  S.anorm = reduce_host(S, n, S.rowsum, nullptr, 1);
+  if (layout == kColMajor) {   // ||A||_inf of the original matrix
+    hipLaunchKernelGGL(rowsum_abs_kernel, dim3(blocks(n)), dim3(256), 0, s, layout, n, A, lda, S.rowsum);
+    ZH_LAUNCH_CHECK();
+    S.anorm = reduce_host(S, n, S.rowsum, nullptr, 1);
+  }
 
   // ---- 2. factorization
   ZH_CHECK(hipMemsetAsync(S.info, 0, sizeof(int), s));
