@@ -1246,12 +1246,13 @@ __global__ void diag_scale_kernel(int64_t n, const double* __restrict__ S, doubl
 }
 
 // dst (kb x kb, FP64, ld ldd) = src (FP32, ld lds) + shift I
+// dst[i][j] = src[i][j] (trans: dst[j][i]) + shift on the diagonal, for a kb x kb block
 __global__ void block_to_f64_kernel(int64_t kb, const float* __restrict__ src, int64_t lds,
-                                    double* __restrict__ dst, int64_t ldd, double shift) {
+                                    double* __restrict__ dst, int64_t ldd, double shift, int trans = 0) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= kb * kb) return;
   const int64_t i = e / kb, j = e % kb;
-  dst[i * ldd + j] = (double)src[i * lds + j] + (i == j ? shift : 0.0);
+  dst[trans ? j * ldd + i : i * ldd + j] = (double)src[i * lds + j] + (i == j ? shift : 0.0);
 }
 
 // dst[i][j] = fl32(trans ? src[j][i] : src[i][j]) for a kb x kb block; lower_only
@@ -1636,15 +1637,16 @@ void chol_factor(Solver& S, const CholWork& cw, int prec, double shift) {
     const int64_t jb = std::min<int64_t>(NB_MIX, n - j0), m2 = n - j0 - jb;
     const int par = (int)(b & 1);
     float* Wd = S.W + j0 * n + j0;
+    // the block as column-major lower (its transposed copy): that factorization's
+    // triangular solves multiply by reciprocals instead of dividing
     hipLaunchKernelGGL(block_to_f64_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb, (const float*)Wd, n,
-                       cw.d64, jb, shift);
+                       cw.d64, jb, shift, 1);
     ZH_LAUNCH_CHECK();
-    // row-major lower = column-major upper of the same memory
-    potrf_f64_cm(kUpper, jb, cw.d64, jb, cw.info_blk + b, sp);
+    potrf_f64_cm(kLower, jb, cw.d64, jb, cw.info_blk + b, sp);
     float* invL = S.invL + b * NB_MIX * NB_MIX;
     float* invLT = S.invLT + b * NB_MIX * NB_MIX;
     hipLaunchKernelGGL(block_to_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, sp, jb, (const double*)cw.d64, jb,
-                       Wd, n, 0, 1);
+                       Wd, n, 1, 1);
     ZH_LAUNCH_CHECK();
     // the FP32 inverse of the rounded factor (as the LU panels invert U11), and its transpose
     tri_inverse_f32(jb, Wd, n, invL, 0, sp);
