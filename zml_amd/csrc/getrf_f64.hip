@@ -106,9 +106,12 @@ template void leaf_factor<float>(int64_t, float*, int64_t, int64_t, int64_t, int
 // Interchanges ipiv[k1..k2] (ascending) applied to `ncols` columns starting
 // at `a` of a row-major matrix, through the composed permutation. Each
 // stream that composes permutations uses its own workspace slot.
+int32_t* perm_pairs_buffer(int slot) {
+  return static_cast<int32_t*>(workspace(sizeof(int32_t) * (2 * panel::PERM_MAX + 4), slot));
+}
 int32_t* compose_perm(int k1, int k2, const int32_t* ipiv, hipStream_t s, int slot) {
   if (k2 - k1 + 1 > panel::PERM_MAX / 2) throw std::runtime_error("compose_perm: block too wide");
-  int32_t* pairs = static_cast<int32_t*>(workspace(sizeof(int32_t) * (2 * panel::PERM_MAX + 4), slot));
+  int32_t* pairs = perm_pairs_buffer(slot);
   hipLaunchKernelGGL(panel::compose_perm_kernel, dim3(1), dim3(64), 0, s, k1, k2, ipiv, pairs);
   ZH_LAUNCH_CHECK();
   return pairs;

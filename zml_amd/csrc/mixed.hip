@@ -240,8 +240,13 @@ __global__ void transpose_f32_kernel(int64_t kb, const float* __restrict__ src, 
 // and columns tx + 32 j (j < 8). Fewer, fuller threads cut the per-column
 // bookkeeping every wave repeats (pivot keys, multipliers). Row interchanges are virtual: a
 // chosen pivot row is marked done and the elimination simply skips done rows;
-// the block is written back in pivot order at the end, and ipiv gets the
-// LAPACK form (sequential interchanges, 1-based, offset by `base`). Pivot =
+// the block is written back in pivot order at the end. ipiv[k] receives the
+// block row that took position k (rowmap_to_ipiv_kernel turns that into the
+// LAPACK form later, off the panel's critical path), and the same order goes
+// out as a list of row moves (destination, source; global rows = block row +
+// `base`), the form apply_perm_kernel reads: into `pairs`, and into the block
+// column's list `bpairs` when given, after the moves already there if
+// `bappend` (the block's rows follow those of the earlier blocks). Pivot =
 // largest |a| among the rows not yet chosen, ties to the lowest row of the
 // block. l = a * fl(1/p) when |p| >= FLT_MIN, else a / p (the rule of zml's
 // getrf2.zig:65-87). A zero or NaN pivot sets *fail.
@@ -261,11 +266,14 @@ __device__ __forceinline__ int key_row(uint64_t k) {
 template <int TY>
 __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__ P, int64_t ld,
                                                             int32_t* __restrict__ ipiv, int base,
-                                                            int* __restrict__ fail) {
+                                                            int* __restrict__ fail, int32_t* __restrict__ pairs,
+                                                            int32_t* __restrict__ bpairs, int bappend) {
   constexpr int TX = 32, RI = DB / TY, RJ = DB / TX, NT = TY * TX, NWV = NT / 64;
+  static_assert(NT >= DB, "one thread per block row in the epilogue");
   __shared__ uint64_t wbest[NWV];
   __shared__ float colk[DB], urow[DB];
-  __shared__ int prow[DB], invp[DB], at[DB], pos[DB];
+  __shared__ int prow[DB], invp[DB];
+  __shared__ int nmove[DB / 64], bfirst;
   __shared__ int bad;
   const int t = threadIdx.x, tx = t % TX, ty = t / TX, lane = t & 63, wv = t >> 6;
   float a[RI][RJ];
@@ -356,12 +364,57 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
       // barrier, which every thread reaches after finishing its reads here
     }
   }
-  // row r of the block goes to position invp[r]; ipiv by replaying the interchanges
+  // row r of the block goes to position invp[r]; position k moves in from
+  // block row prow[k] when they differ (moves numbered in position order)
+  const bool moved = t < DB && prow[t] != t;
+  const uint64_t mb = __ballot(moved);
   if (t < DB) {
     invp[prow[t]] = t;
-    at[t] = t;
-    pos[t] = t;
+    ipiv[t] = prow[t];
+    if (lane == 0) nmove[wv] = (int)__popcll(mb);
   }
+  if (t == 0) {
+    bfirst = bpairs && bappend ? bpairs[0] : 0;
+    if (bad) atomicOr(fail, 1);
+  }
+  __syncthreads();
+  if (t < DB) {
+    int idx = (int)__popcll(mb & ((1ull << lane) - 1)), total = 0;
+#pragma unroll
+    for (int v = 0; v < DB / 64; ++v) {
+      idx += v < wv ? nmove[v] : 0;
+      total += nmove[v];
+    }
+    if (moved) {
+      pairs[1 + 2 * idx] = base + t;
+      pairs[2 + 2 * idx] = base + prow[t];
+      if (bpairs) {
+        bpairs[1 + 2 * (bfirst + idx)] = base + t;
+        bpairs[2 + 2 * (bfirst + idx)] = base + prow[t];
+      }
+    }
+    if (t == 0) {
+      pairs[0] = total;
+      if (bpairs) bpairs[0] = bfirst + total;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < RI; ++i) {
+    const int64_t row = invp[ty + TY * i];
+#pragma unroll
+    for (int j = 0; j < RJ; ++j) P[row * ld + tx + TX * j] = a[i][j];
+  }
+}
+
+// ipiv[0..DB) holds a block's row order (position k took block row ipiv[k]);
+// rewritten in place in the LAPACK form: the interchanges that, applied in
+// order k = 0, 1, ..., produce that order (1-based, offset by `base`).
+__global__ void __launch_bounds__(DB) rowmap_to_ipiv_kernel(int32_t* __restrict__ ipiv, int base) {
+  __shared__ int prow[DB], at[DB], pos[DB];   // at: row at a position; pos: position of a row
+  const int t = threadIdx.x;
+  prow[t] = ipiv[t];
+  at[t] = t;
+  pos[t] = t;
   __syncthreads();
   if (t == 0) {
     for (int k = 0; k < DB; ++k) {
@@ -372,13 +425,6 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
       at[k] = p;
       pos[p] = k;
     }
-    if (bad) atomicOr(fail, 1);
-  }
-#pragma unroll
-  for (int i = 0; i < RI; ++i) {
-    const int64_t row = invp[ty + TY * i];
-#pragma unroll
-    for (int j = 0; j < RJ; ++j) P[row * ld + tx + TX * j] = a[i][j];
   }
 }
 
@@ -711,36 +757,63 @@ __global__ void __launch_bounds__(64) tri_inv64_kernel(const float* __restrict__
 // X (ld ldx) <- inverse of a 256 x 256 triangular matrix in one workgroup.
 // !TRANS: L lower (ld ld), X = L^-1. TRANS: the input is an upper U; the
 // kernel inverts L = U^T (reading U transposed) and writes X^T = U^-1.
-// Phase A: wave w inverts the 64 x 64 diagonal block w (lane j solves for
+// Phase A: wave w < 4 inverts the 64 x 64 diagonal block w (lane j solves for
 // column j, the block staged in LDS). Phase B, column block J = 0..2 and row
 // block I = J+1..3 in order: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ, as 64 x 64
-// products over LDS (thread: 4 x 4 outputs). Only X's lower blocks are
-// written; the caller zeroes the rest.
+// products over LDS (thread: TR x TC outputs, each a k-ordered FMA chain).
+// Global memory is read and written in whole blocks, numbered so that
+// consecutive threads touch consecutive addresses in either orientation: the
+// four diagonal blocks are loaded together, and each L block of phase B is
+// loaded into registers while the previous product runs. Only X's lower
+// blocks are written; the caller zeroes the rest.
 constexpr int TI_LD = 65;
+constexpr int TI_NT = 512, TI_TR = 4, TI_TC = 2;
 template <bool UNIT, bool TRANS>
-__global__ void __launch_bounds__(256) tri_inv256_kernel(const float* __restrict__ L, int64_t ld,
-                                                         float* __restrict__ X, int64_t ldx) {
+__global__ void __launch_bounds__(TI_NT) tri_inv256_kernel(const float* __restrict__ L, int64_t ld,
+                                                           float* __restrict__ X, int64_t ldx) {
+  constexpr int NT = TI_NT, TR = TI_TR, TC = TI_TC, CT = 64 / TC;
+  static_assert((64 / TR) * CT == NT, "one output tile per thread");
+  constexpr int SA = 4 * 4096 / NT;   // elements per thread of the four diagonal blocks
+  constexpr int SB = 4096 / NT;       // of one 64 x 64 block
   __shared__ float smem[9 * 64 * TI_LD];
   float* Xd = smem;                       // 4 diagonal inverses, 64 x TI_LD each
   float* Xo = Xd + 4 * 64 * TI_LD;        // this column block's X_KJ, K > J (3)
   float* Ls = Xo + 3 * 64 * TI_LD;        // L_IK staging
   float* Tm = Ls + 64 * TI_LD;            // sum of products
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  // element (r, c) of the lower factor, block (I, K)
-  auto lget = [&](int I, int K, int r, int c) -> float {
-    const int64_t gr = I * 64 + r, gc = K * 64 + c;
-    return TRANS ? L[gc * ld + gr] : L[gr * ld + gc];
+  // element e of a 64 x 64 block: row er(e), column ec(e)
+  auto er = [](int e) { return TRANS ? (e & 63) : (e >> 6); };
+  auto ec = [](int e) { return TRANS ? (e >> 6) : (e & 63); };
+  auto laddr = [&](int I, int K, int e) -> int64_t {
+    const int64_t gr = I * 64 + er(e), gc = K * 64 + ec(e);
+    return TRANS ? gc * ld + gr : gr * ld + gc;
   };
-  auto xput = [&](int I, int J, int r, int c, float v) {
-    const int64_t gr = I * 64 + r, gc = J * 64 + c;
-    if (TRANS) X[gc * ldx + gr] = v;
-    else X[gr * ldx + gc] = v;
+  auto xaddr = [&](int I, int J, int e) -> int64_t {
+    const int64_t gr = I * 64 + er(e), gc = J * 64 + ec(e);
+    return TRANS ? gc * ldx + gr : gr * ldx + gc;
   };
-  // ---- phase A (Xo is free yet: it stages the four diagonal blocks)
+  // ---- phase A (Xo is free yet: it stages the four diagonal blocks, 4 fit in Xo + Ls)
   {
-    float* st = Xo + w * 64 * TI_LD;   // 4 blocks fit in Xo + Ls
-    for (int e = lane; e < 64 * 64; e += 64) st[(e >> 6) * TI_LD + (e & 63)] = lget(w, w, e >> 6, e & 63);
-    __syncthreads();
+    float v[SA];
+#pragma unroll
+    for (int q = 0; q < SA; ++q) {
+      const int f = t + NT * q, b = f >> 12, e = f & 4095;
+      v[q] = L[laddr(b, b, e)];
+    }
+#pragma unroll
+    for (int q = 0; q < SA; ++q) {
+      const int f = t + NT * q, b = f >> 12, e = f & 4095;
+      Xo[(b * 64 + er(e)) * TI_LD + ec(e)] = v[q];
+    }
+  }
+  float pf[SB];   // the next L block of phase B
+  auto load_l = [&](int I, int K) {
+#pragma unroll
+    for (int q = 0; q < SB; ++q) pf[q] = L[laddr(I, K, t + NT * q)];
+  };
+  __syncthreads();
+  if (w < 4) {
+    const float* st = Xo + w * 64 * TI_LD;
     float x[64];
 #pragma unroll
     for (int k = 0; k < 64; ++k) {
@@ -751,70 +824,90 @@ __global__ void __launch_bounds__(256) tri_inv256_kernel(const float* __restrict
       x[k] = UNIT ? acc : acc / row[k];
     }
 #pragma unroll
-    for (int k = 0; k < 64; ++k) {
-      Xd[(w * 64 + k) * TI_LD + lane] = x[k];
-      xput(w, w, k, lane, x[k]);
-    }
+    for (int k = 0; k < 64; ++k) Xd[(w * 64 + k) * TI_LD + lane] = x[k];
   }
+  load_l(1, 0);
   __syncthreads();
+#pragma unroll
+  for (int q = 0; q < SA; ++q) {
+    const int f = t + NT * q, b = f >> 12, e = f & 4095;
+    X[xaddr(b, b, e)] = Xd[(b * 64 + er(e)) * TI_LD + ec(e)];
+  }
   // ---- phase B
-  const int r0 = (t >> 4) * 4, c0 = (t & 15) * 4;
+  const int r0 = (t / CT) * TR, c0 = (t % CT) * TC;
   for (int J = 0; J < 3; ++J) {
     for (int I = J + 1; I < 4; ++I) {
-      float acc[4][4];
+      float acc[TR][TC];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < TR; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+        for (int b = 0; b < TC; ++b) acc[a][b] = 0.0f;
       for (int K = J; K < I; ++K) {
-        for (int e = t; e < 64 * 64; e += 256) Ls[(e >> 6) * TI_LD + (e & 63)] = lget(I, K, e >> 6, e & 63);
+#pragma unroll
+        for (int q = 0; q < SB; ++q) {
+          const int e = t + NT * q;
+          Ls[er(e) * TI_LD + ec(e)] = pf[q];
+        }
+        // the L block after this one: (I, K+1), else (I+1, J), else (J+2, J+1)
+        int nI = I, nK = K + 1;
+        if (nK == I) {
+          nI = I + 1;
+          nK = J;
+          if (nI == 4) {
+            nI = J + 2;
+            nK = J + 1;
+          }
+        }
+        if (nI < 4) load_l(nI, nK);
         __syncthreads();
         const float* Q = K == J ? Xd + J * 64 * TI_LD : Xo + (K - J - 1) * 64 * TI_LD;
 #pragma unroll 8
         for (int k = 0; k < 64; ++k) {
-          float pa[4], qb[4];
+          float pa[TR], qb[TC];
 #pragma unroll
-          for (int a = 0; a < 4; ++a) pa[a] = Ls[(r0 + a) * TI_LD + k];
+          for (int a = 0; a < TR; ++a) pa[a] = Ls[(r0 + a) * TI_LD + k];
 #pragma unroll
-          for (int b = 0; b < 4; ++b) qb[b] = Q[k * TI_LD + c0 + b];
+          for (int b = 0; b < TC; ++b) qb[b] = Q[k * TI_LD + c0 + b];
 #pragma unroll
-          for (int a = 0; a < 4; ++a)
+          for (int a = 0; a < TR; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(pa[a], qb[b], acc[a][b]);
+            for (int b = 0; b < TC; ++b) acc[a][b] = fmaf(pa[a], qb[b], acc[a][b]);
         }
         __syncthreads();
       }
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < TR; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) Tm[(r0 + a) * TI_LD + c0 + b] = acc[a][b];
+        for (int b = 0; b < TC; ++b) Tm[(r0 + a) * TI_LD + c0 + b] = acc[a][b];
       __syncthreads();
       const float* D = Xd + I * 64 * TI_LD;
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < TR; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+        for (int b = 0; b < TC; ++b) acc[a][b] = 0.0f;
 #pragma unroll 8
       for (int k = 0; k < 64; ++k) {
-        float pa[4], qb[4];
+        float pa[TR], qb[TC];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) pa[a] = D[(r0 + a) * TI_LD + k];
+        for (int a = 0; a < TR; ++a) pa[a] = D[(r0 + a) * TI_LD + k];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) qb[b] = Tm[k * TI_LD + c0 + b];
+        for (int b = 0; b < TC; ++b) qb[b] = Tm[k * TI_LD + c0 + b];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < TR; ++a)
 #pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(pa[a], qb[b], acc[a][b]);
+          for (int b = 0; b < TC; ++b) acc[a][b] = fmaf(pa[a], qb[b], acc[a][b]);
       }
       float* O = Xo + (I - J - 1) * 64 * TI_LD;
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < TR; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          O[(r0 + a) * TI_LD + c0 + b] = -acc[a][b];
-          xput(I, J, r0 + a, c0 + b, -acc[a][b]);
-        }
+        for (int b = 0; b < TC; ++b) O[(r0 + a) * TI_LD + c0 + b] = -acc[a][b];
       __syncthreads();
+#pragma unroll
+      for (int q = 0; q < SB; ++q) {
+        const int e = t + NT * q;
+        X[xaddr(I, J, e)] = O[er(e) * TI_LD + ec(e)];
+      }
     }
   }
 }
@@ -824,11 +917,11 @@ __global__ void __launch_bounds__(256) tri_inv256_kernel(const float* __restrict
 // already be zero (lower part for trans).
 void tri_inv256(const float* L, int64_t ld, float* X, int64_t ldx, int unit, hipStream_t s, bool trans = false) {
   if (trans) {
-    if (unit) hipLaunchKernelGGL((tri_inv256_kernel<true, true>), dim3(1), dim3(256), 0, s, L, ld, X, ldx);
-    else hipLaunchKernelGGL((tri_inv256_kernel<false, true>), dim3(1), dim3(256), 0, s, L, ld, X, ldx);
+    if (unit) hipLaunchKernelGGL((tri_inv256_kernel<true, true>), dim3(1), dim3(TI_NT), 0, s, L, ld, X, ldx);
+    else hipLaunchKernelGGL((tri_inv256_kernel<false, true>), dim3(1), dim3(TI_NT), 0, s, L, ld, X, ldx);
   } else {
-    if (unit) hipLaunchKernelGGL((tri_inv256_kernel<true, false>), dim3(1), dim3(256), 0, s, L, ld, X, ldx);
-    else hipLaunchKernelGGL((tri_inv256_kernel<false, false>), dim3(1), dim3(256), 0, s, L, ld, X, ldx);
+    if (unit) hipLaunchKernelGGL((tri_inv256_kernel<true, false>), dim3(1), dim3(TI_NT), 0, s, L, ld, X, ldx);
+    else hipLaunchKernelGGL((tri_inv256_kernel<false, false>), dim3(1), dim3(TI_NT), 0, s, L, ld, X, ldx);
   }
   ZH_LAUNCH_CHECK();
 }
@@ -910,26 +1003,29 @@ void panel_product_f32(int64_t m, int64_t n, int64_t K, float alpha, const float
 // pivot (1) or an |l| above the threshold (2).
 // linv (optional): the inverse of the block's unit lower factor is formed there
 // (its strictly upper part already zero) on a second stream, joined by the
-// event aux_event(5), while U11^-1 and L21 take this stream.
-// perm_k1 >= 0 (the block column's last diagonal block): its pivots complete
-// the block column's, whose interchanges are then composed on the second stream
-// too, into the buffer of the block column's parity (tl_mix.block_pairs).
+// event aux_event(5), while U11^-1 and L21 take this stream; the block's
+// ipiv entries reach their LAPACK form there too (until then they hold the
+// block's row order). Without linv that happens on this stream.
+// The block's row moves go to `pairs`, and to the block column's list
+// `bpairs` when given (appended if `bappend`).
 void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, int64_t src_ld, int32_t* ipiv,
-                      int base, float* uinv, int64_t uinv_ld, hipStream_t s, float* linv = nullptr,
-                      int64_t linv_ld = 0, int perm_k1 = -1, const int32_t* perm_ipiv = nullptr) {
+                      int base, float* uinv, int64_t uinv_ld, hipStream_t s, int32_t* pairs,
+                      float* linv = nullptr, int64_t linv_ld = 0, int32_t* bpairs = nullptr, int bappend = 0) {
   MixedCtx& mx = tl_mix;
-  hipLaunchKernelGGL(diag_getf2_kernel<DIAG_TY>, dim3(1), dim3(DIAG_TY * 32), 0, s, P, lda, ipiv, base, mx.pfail);
+  hipLaunchKernelGGL(diag_getf2_kernel<DIAG_TY>, dim3(1), dim3(DIAG_TY * 32), 0, s, P, lda, ipiv, base, mx.pfail,
+                     pairs, bpairs, bappend);
   ZH_LAUNCH_CHECK();
   if (linv) {
     const hipStream_t s2 = aux_stream(2);
     ZH_CHECK(hipEventRecord(aux_event(4), s));
     ZH_CHECK(hipStreamWaitEvent(s2, aux_event(4), 0));
     tri_inv256(P, lda, linv, linv_ld, 1, s2);
-    if (perm_k1 >= 0) {
-      const int par = (perm_k1 / NB_MIX) & 1;
-      mx.block_pairs[par] = compose_perm(perm_k1, perm_k1 + NB_MIX - 1, perm_ipiv, s2, 19 + par);
-    }
+    hipLaunchKernelGGL(rowmap_to_ipiv_kernel, dim3(1), dim3(DB), 0, s2, ipiv, base);
+    ZH_LAUNCH_CHECK();
     ZH_CHECK(hipEventRecord(aux_event(5), s2));
+  } else {
+    hipLaunchKernelGGL(rowmap_to_ipiv_kernel, dim3(1), dim3(DB), 0, s, ipiv, base);
+    ZH_LAUNCH_CHECK();
   }
   // U11^-1 (upper) into scr1, whose strictly lower part must be zero
   hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr1,
@@ -983,11 +1079,15 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, linv, jb);
     ZH_LAUNCH_CHECK();
   }
-  optimistic_block(P, lda, rows, mx.pbak + DB * jb, jb, ipiv + j0, (int)j0, ustore, NB_MIX, s, linv, jb);
+  // row moves: each half's own list, and the block column's (both halves) for
+  // the driver, in the buffer of the block column's parity
+  int32_t* pr = perm_pairs_buffer(2);
+  const int par = (int)((j0 / NB_MIX) & 1);
+  int32_t* bpr = jb == NB_MIX ? perm_pairs_buffer(19 + par) : nullptr;
+  optimistic_block(P, lda, rows, mx.pbak + DB * jb, jb, ipiv + j0, (int)j0, ustore, NB_MIX, s, pr, linv, jb, bpr, 0);
   if (jb == 2 * DB) {
     const int64_t c2 = j0 + DB;                 // second half: columns c2 .. c2 + DB
     const int64_t rows2 = rows - DB;
-    const int32_t* pr = compose_perm((int)j0, (int)(c2 - 1), ipiv, s, 2);
     apply_perm<float>(DB, A + c2, lda, pr, s);
     ZH_CHECK(hipStreamWaitEvent(s, aux_event(5), 0));   // La^-1
     // U12 of the first half = La^-1 A12, then A22 -= L21 U12 within the block column
@@ -1000,9 +1100,8 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
       ZH_CHECK(hipMemcpy2DAsync(mx.ptmp, sizeof(float) * DB, A + (c2 + DB) * lda + c2, sizeof(float) * lda,
                                 sizeof(float) * DB, rows2 - DB, hipMemcpyDeviceToDevice, s));
     optimistic_block(A + c2 * lda + c2, lda, rows2, mx.ptmp, DB, ipiv + c2, (int)c2,
-                     ustore + DB * NB_MIX + DB, NB_MIX, s, linv + DB * jb + DB, jb, (int)j0, ipiv);
-    const int32_t* pr2 = compose_perm((int)c2, (int)(c2 + DB - 1), ipiv, s, 2);
-    apply_perm<float>(DB, A + j0, lda, pr2, s);
+                     ustore + DB * NB_MIX + DB, NB_MIX, s, pr, linv + DB * jb + DB, jb, bpr, 1);
+    apply_perm<float>(DB, A + j0, lda, pr, s);
     // the off-diagonal block -Lb^-1 Lba La^-1
     ZH_CHECK(hipStreamWaitEvent(s, aux_event(5), 0));   // Lb^-1
     panel_product_f32(DB, DB, DB, 1.0f, A + c2 * lda + j0, lda, linv, jb, 0.0f, mx.scr3, DB, s);
@@ -1017,7 +1116,8 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     if (jb == NB_MIX) {
       mx.have_invu[j0 / NB_MIX] = 1;
       mx.inv_ready = mx.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;
-      mx.block_pairs_j0[(j0 / NB_MIX) & 1] = j0;
+      mx.block_pairs[par] = bpr;
+      mx.block_pairs_j0[par] = j0;
     }
     return true;
   }

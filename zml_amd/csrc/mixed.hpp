@@ -68,7 +68,10 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
                 hipStream_t s);
 
 // Row interchanges of a block of rows (getrf_f64.hip): compose ipiv[k1..k2]
-// into moves, apply the moves to `ncols` columns starting at `a`.
+// into moves, apply the moves to `ncols` columns starting at `a`. A move list
+// (count, then destination/source row pairs) lives in a workspace slot's
+// buffer, perm_pairs_buffer(slot), of up to 2 x 512 moves.
+int32_t* perm_pairs_buffer(int slot);
 int32_t* compose_perm(int k1, int k2, const int32_t* ipiv, hipStream_t s, int slot);
 template <typename T>
 void apply_perm(int64_t ncols, T* a, int64_t lda, const int32_t* pairs, hipStream_t s);
