@@ -268,9 +268,9 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
                                                             int32_t* __restrict__ ipiv, int base,
                                                             int* __restrict__ fail, int32_t* __restrict__ pairs,
                                                             int32_t* __restrict__ bpairs, int bappend) {
-  constexpr int TX = 32, RI = DB / TY, RJ = DB / TX, NT = TY * TX, NWV = NT / 64;
+  constexpr int TX = 32, RI = DB / TY, RJ = DB / TX, NT = TY * TX;
   static_assert(NT >= DB, "one thread per block row in the epilogue");
-  __shared__ uint64_t wbest[NWV];
+  __shared__ unsigned long long gkey[2];   // pivot keys of consecutive columns
   __shared__ float colk[DB], urow[DB];
   __shared__ int prow[DB], invp[DB];
   __shared__ int nmove[DB / 64], bfirst;
@@ -281,8 +281,12 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
   for (int i = 0; i < RI; ++i)
 #pragma unroll
     for (int j = 0; j < RJ; ++j) a[i][j] = P[(int64_t)(ty + TY * i) * ld + tx + TX * j];
-  if (t == 0) bad = 0;
+  if (t == 0) {
+    bad = 0;
+    gkey[0] = 0;
+  }
   unsigned done = 0;
+  __syncthreads();
 #pragma unroll
   for (int jj = 0; jj < RJ; ++jj) {
     for (int kq = 0; kq < TX; ++kq) {
@@ -299,19 +303,17 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
           }
         }
       }
+      // the workgroup's pivot: each wave's best key meets the others in an LDS
+      // maximum (two slots: the next column's is cleared now, its last readers
+      // having passed this column's second barrier)
       const uint64_t other = __shfl_xor(best, 32);
       best = other > best ? other : best;
-      if (lane == kq) wbest[wv] = best;
+      if (lane == kq) atomicMax(&gkey[k & 1], (unsigned long long)best);
+      if (t == 0) gkey[(k + 1) & 1] = 0;
       __syncthreads();
-      // the workgroup's pivot: lanes 0 .. NWV - 1 of every wave reduce the wave keys
-      uint64_t g = lane < NWV ? wbest[lane] : 0;
-#pragma unroll
-      for (int off = 1; off < NWV; off <<= 1) {
-        const uint64_t o = __shfl_xor(g, off);
-        g = o > g ? o : g;
-      }
-      g = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(g >> 32), 0) << 32) |
-          (uint32_t)__builtin_amdgcn_readlane((uint32_t)g, 0);
+      uint64_t g = gkey[k & 1];
+      g = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32) |
+          (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)g);
       const int p = key_row(g), pt = p % TY, pi = p / TY;
       const float pv = colk[p];
       const bool ok = pv != 0.0f && pv == pv;
@@ -360,8 +362,8 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
           for (int i = 0; i < RI; ++i) a[i][j] -= li[i] * u;
         }
       }
-      // colk, urow and wbest are rewritten only after the next step's first
-      // barrier, which every thread reaches after finishing its reads here
+      // colk and urow are rewritten only after the next step's first barrier,
+      // which every thread reaches after finishing its reads here
     }
   }
   // row r of the block goes to position invp[r]; position k moves in from
