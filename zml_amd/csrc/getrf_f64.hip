@@ -171,16 +171,24 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
   // copy of A12 is used by the main stream only
   T* inv = static_cast<T*>(workspace(sizeof(T) * 2 * nb * nb, 0));
   T* tmp = static_cast<T*>(workspace(sizeof(T) * nb * (size_t)std::max<int64_t>(n, 1), 1));
-  auto panel_and_inverse = [&](int64_t j0, int64_t jb, int64_t blk) {
-    if (!(ops.try_panel && ops.try_panel(m, A, lda, j0, jb, ipiv, sp)))
-      factor_panel<T>(m, A, lda, j0, jb, ipiv, info, ops, sp, 2);
+  // a panel in two parts: the optimistic attempt is queued first; its outcome
+  // (and the full pivot search after a failed one) and the inverse follow
+  auto panel_begin = [&](int64_t j0, int64_t jb) {
+    return ops.try_panel && ops.try_panel(m, A, lda, j0, jb, ipiv, sp);
+  };
+  auto panel_end = [&](int64_t j0, int64_t jb, int64_t blk, bool attempted) {
+    const bool ok = attempted && (!ops.try_panel_finish || ops.try_panel_finish(j0, jb, A, lda, sp));
+    if (!ok) factor_panel<T>(m, A, lda, j0, jb, ipiv, info, ops, sp, 2);
     T* invp = ops.inv_store ? ops.inv_store(j0) : inv + (blk & 1) * nb * nb;
     if (j0 + jb < n || ops.inv_store) ops.invert(jb, A + j0 * lda + j0, lda, invp, sp);
     ZH_CHECK(hipEventRecord(ev_panel, sp));
   };
   ZH_CHECK(hipEventRecord(ev_in, s));
   ZH_CHECK(hipStreamWaitEvent(sp, ev_in, 0));
-  panel_and_inverse(0, std::min(nb, mn), 0);
+  {
+    const int64_t jb0 = std::min(nb, mn);
+    panel_end(0, jb0, 0, panel_begin(0, jb0));
+  }
   for (int64_t j0 = 0, blk = 0; j0 < mn; j0 += nb, ++blk) {
     const int64_t jb = std::min(nb, mn - j0);
     const T* invb = ops.inv_store ? ops.inv_store(j0) : inv + (blk & 1) * nb * nb;
@@ -197,8 +205,16 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
       if (rows > 0) ops.gemm(rows, jb2, jb, A + c1 * lda + j0, lda, A + j0 * lda + c1, lda, A + c1 * lda + c1, lda, s);
       ZH_CHECK(hipEventRecord(ev_next, s));
     }
+    // a split panel's attempt goes ahead of this step's remaining work (the
+    // host then queues that work while the panel runs)
+    bool begun = false, attempted = false;
+    if (jb2 > 0 && ops.try_panel_finish) {
+      ZH_CHECK(hipStreamWaitEvent(sp, ev_next, 0));
+      attempted = panel_begin(c1, jb2);
+      begun = true;
+    }
     // the rest: interchanges left of the block and beyond the next one, then
-    // update (queued before the next panel: an optimistic panel waits on the host)
+    // update (queued before the next panel's outcome, which waits on the host)
     apply_perm<T>(j0, A, lda, pr, s);
     const int64_t c2 = c1 + jb2, nr = n - c2;
     if (nr > 0) {
@@ -207,8 +223,11 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
       if (rows > 0) ops.gemm(rows, nr, jb, A + c1 * lda + j0, lda, A + j0 * lda + c2, lda, A + c1 * lda + c2, lda, s);
     }
     if (jb2 > 0) {
-      ZH_CHECK(hipStreamWaitEvent(sp, ev_next, 0));
-      panel_and_inverse(c1, jb2, blk + 1);
+      if (!begun) {
+        ZH_CHECK(hipStreamWaitEvent(sp, ev_next, 0));
+        attempted = panel_begin(c1, jb2);
+      }
+      panel_end(c1, jb2, blk + 1, attempted);
     }
   }
   if (s != caller) {

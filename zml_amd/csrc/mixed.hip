@@ -658,6 +658,8 @@ struct MixedCtx {
   float* tmp = nullptr;      // inverse scratch
   const float* l16_of = nullptr;   // L21 whose rounded copy l16 holds (one block column's updates share it)
   const int32_t* block_pairs[2] = {nullptr, nullptr};   // composed interchanges by block-column parity
+  int32_t* pending_pairs = nullptr;                      // the queued panel attempt's: its list
+  int64_t pending_rows = 0;                              // ... and its row count
   int64_t block_pairs_j0[2] = {-1, -1};                 // ... of the block column starting here
   int64_t l16_rows = 0, l16_kb = 0;
   // optimistic panels
@@ -762,19 +764,20 @@ __global__ void __launch_bounds__(64) tri_inv64_kernel(const float* __restrict__
 // Phase A: wave w < 4 inverts the 64 x 64 diagonal block w (lane j solves for
 // column j, the block staged in LDS). Phase B, column block J = 0..2 and row
 // block I = J+1..3 in order: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ, as 64 x 64
-// products over LDS (thread: TR x TC outputs, each a k-ordered FMA chain).
+// products of LDS operands on the matrix cores (waves 0..3, one 32 x 32 tile
+// each, 32x32x2 FP32 MFMA steps in k order).
 // Global memory is read and written in whole blocks, numbered so that
 // consecutive threads touch consecutive addresses in either orientation: the
 // four diagonal blocks are loaded together, and each L block of phase B is
 // loaded into registers while the previous product runs. Only X's lower
 // blocks are written; the caller zeroes the rest.
 constexpr int TI_LD = 65;
-constexpr int TI_NT = 512, TI_TR = 4, TI_TC = 2;
+constexpr int TI_NT = 512;
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 template <bool UNIT, bool TRANS>
 __global__ void __launch_bounds__(TI_NT) tri_inv256_kernel(const float* __restrict__ L, int64_t ld,
                                                            float* __restrict__ X, int64_t ldx) {
-  constexpr int NT = TI_NT, TR = TI_TR, TC = TI_TC, CT = 64 / TC;
-  static_assert((64 / TR) * CT == NT, "one output tile per thread");
+  constexpr int NT = TI_NT;
   constexpr int SA = 4 * 4096 / NT;   // elements per thread of the four diagonal blocks
   constexpr int SB = 4096 / NT;       // of one 64 x 64 block
   __shared__ float smem[9 * 64 * TI_LD];
@@ -835,15 +838,23 @@ __global__ void __launch_bounds__(TI_NT) tri_inv256_kernel(const float* __restri
     const int f = t + NT * q, b = f >> 12, e = f & 4095;
     X[xaddr(b, b, e)] = Xd[(b * 64 + er(e)) * TI_LD + ec(e)];
   }
-  // ---- phase B
-  const int r0 = (t / CT) * TR, c0 = (t % CT) * TC;
+  // ---- phase B: wave w < 4 holds the 32 x 32 tile (mi, mj) of each product,
+  // element r of lane l at row mi + mrow(r), column mj + l % 32
+  const int mi = 32 * ((w & 3) >> 1), mj = 32 * (w & 1);
+  auto mrow = [&](int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); };
+  f32x16 mac;
+  auto mma64 = [&](const float* Am, const float* Bm) {   // mac += Am Bm (64 x 64, ld TI_LD)
+#pragma unroll 8
+    for (int ks = 0; ks < 32; ++ks) {
+      const float av = Am[(mi + (lane & 31)) * TI_LD + 2 * ks + (lane >> 5)];
+      const float bv = Bm[(2 * ks + (lane >> 5)) * TI_LD + mj + (lane & 31)];
+      mac = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, mac, 0, 0, 0);
+    }
+  };
   for (int J = 0; J < 3; ++J) {
     for (int I = J + 1; I < 4; ++I) {
-      float acc[TR][TC];
 #pragma unroll
-      for (int a = 0; a < TR; ++a)
-#pragma unroll
-        for (int b = 0; b < TC; ++b) acc[a][b] = 0.0f;
+      for (int r = 0; r < 16; ++r) mac[r] = 0.0f;
       for (int K = J; K < I; ++K) {
 #pragma unroll
         for (int q = 0; q < SB; ++q) {
@@ -863,47 +874,22 @@ __global__ void __launch_bounds__(TI_NT) tri_inv256_kernel(const float* __restri
         if (nI < 4) load_l(nI, nK);
         __syncthreads();
         const float* Q = K == J ? Xd + J * 64 * TI_LD : Xo + (K - J - 1) * 64 * TI_LD;
-#pragma unroll 8
-        for (int k = 0; k < 64; ++k) {
-          float pa[TR], qb[TC];
-#pragma unroll
-          for (int a = 0; a < TR; ++a) pa[a] = Ls[(r0 + a) * TI_LD + k];
-#pragma unroll
-          for (int b = 0; b < TC; ++b) qb[b] = Q[k * TI_LD + c0 + b];
-#pragma unroll
-          for (int a = 0; a < TR; ++a)
-#pragma unroll
-            for (int b = 0; b < TC; ++b) acc[a][b] = fmaf(pa[a], qb[b], acc[a][b]);
-        }
+        if (w < 4) mma64(Ls, Q);
         __syncthreads();
       }
+      if (w < 4) {
 #pragma unroll
-      for (int a = 0; a < TR; ++a)
-#pragma unroll
-        for (int b = 0; b < TC; ++b) Tm[(r0 + a) * TI_LD + c0 + b] = acc[a][b];
-      __syncthreads();
-      const float* D = Xd + I * 64 * TI_LD;
-#pragma unroll
-      for (int a = 0; a < TR; ++a)
-#pragma unroll
-        for (int b = 0; b < TC; ++b) acc[a][b] = 0.0f;
-#pragma unroll 8
-      for (int k = 0; k < 64; ++k) {
-        float pa[TR], qb[TC];
-#pragma unroll
-        for (int a = 0; a < TR; ++a) pa[a] = D[(r0 + a) * TI_LD + k];
-#pragma unroll
-        for (int b = 0; b < TC; ++b) qb[b] = Tm[k * TI_LD + c0 + b];
-#pragma unroll
-        for (int a = 0; a < TR; ++a)
-#pragma unroll
-          for (int b = 0; b < TC; ++b) acc[a][b] = fmaf(pa[a], qb[b], acc[a][b]);
+        for (int r = 0; r < 16; ++r) Tm[(mi + mrow(r)) * TI_LD + mj + (lane & 31)] = mac[r];
       }
+      __syncthreads();
       float* O = Xo + (I - J - 1) * 64 * TI_LD;
+      if (w < 4) {
 #pragma unroll
-      for (int a = 0; a < TR; ++a)
+        for (int r = 0; r < 16; ++r) mac[r] = 0.0f;
+        mma64(Xd + I * 64 * TI_LD, Tm);
 #pragma unroll
-        for (int b = 0; b < TC; ++b) O[(r0 + a) * TI_LD + c0 + b] = -acc[a][b];
+        for (int r = 0; r < 16; ++r) O[(mi + mrow(r)) * TI_LD + mj + (lane & 31)] = -mac[r];
+      }
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < SB; ++q) {
@@ -1053,7 +1039,8 @@ void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, i
 // rounding and ties); with |l_ij| <= tau > 1 it is threshold pivoting, whose
 // growth per step is at most 1 + tau (the FP64 refinement then restores the
 // accuracy, and GMRES-IR or the FP64 factorization take over if it cannot).
-// Otherwise the block column is restored and the caller factors it with the
+// try_panel_f32 queues the attempt; finish_panel_f32 waits for it and, when it
+// failed, restores the block column, which the caller then factors with the
 // full pivot search.
 const int32_t* block_perm_f32(int64_t j0) {
   const MixedCtx& mx = tl_mix;
@@ -1069,6 +1056,7 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
   if (!mx.optimistic || (jb != DB && jb != 2 * DB) || rows < jb) return false;
   ++mx.panels;
   mx.inv_ready = nullptr;
+  mx.pending_rows = rows;
   float* P = A + j0 * lda + j0;
   float* ustore = mx.invu_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;   // ld NB_MIX
   ZH_CHECK(hipMemsetAsync(mx.pfail, 0, sizeof(int), s));
@@ -1109,6 +1097,16 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     panel_product_f32(DB, DB, DB, 1.0f, A + c2 * lda + j0, lda, linv, jb, 0.0f, mx.scr3, DB, s);
     panel_product_f32(DB, DB, DB, -1.0f, linv + DB * jb + DB, jb, mx.scr3, DB, 0.0f, linv + DB * jb, jb, s);
   }
+  mx.pending_pairs = bpr;
+  return true;
+}
+
+// The outcome of the attempt try_panel_f32 queued for block column j0.
+bool finish_panel_f32(int64_t j0, int64_t jb, float* A, int64_t lda, hipStream_t s) {
+  MixedCtx& mx = tl_mix;
+  float* P = A + j0 * lda + j0;
+  const int64_t rows = mx.pending_rows;
+  const int par = (int)((j0 / NB_MIX) & 1);
   int fail = 0;
   ZH_CHECK(hipMemcpyAsync(&fail, mx.pfail, sizeof(int), hipMemcpyDeviceToHost, s));
   ZH_CHECK(hipStreamSynchronize(s));
@@ -1118,7 +1116,7 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
     if (jb == NB_MIX) {
       mx.have_invu[j0 / NB_MIX] = 1;
       mx.inv_ready = mx.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;
-      mx.block_pairs[par] = bpr;
+      mx.block_pairs[par] = mx.pending_pairs;
       mx.block_pairs_j0[par] = j0;
     }
     return true;
@@ -1996,7 +1994,7 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
     static float* at(int64_t j0) { return tl_mix.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX; }
   };
   const LuOps<float> ops{trailing_gemm, trsm_f32, Inv::store, apply_inv_f32, panel_gemm_f32, try_panel_f32,
-                         InvAt::at, 8, block_perm_f32};
+                         InvAt::at, 8, block_perm_f32, finish_panel_f32};
   getrf_lookahead<float>(n, n, S.W, n, S.ipiv, S.info, NB_MIX, ops, s);
   int info = 0;
   ZH_CHECK(hipMemcpyAsync(&info, S.info, sizeof(int), hipMemcpyDeviceToHost, s));

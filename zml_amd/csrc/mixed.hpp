@@ -57,6 +57,11 @@ struct LuOps {
   // optional: the composed interchanges of block column j0 when the panel
   // already formed them (the driver composes them itself when null)
   const int32_t* (*block_perm)(int64_t j0);
+  // optional, with try_panel: then try_panel only queues its attempt (its
+  // return value says whether it made one), and this reports the outcome,
+  // waiting for the panel stream; a failed attempt leaves the block column as
+  // it was. The driver queues the previous step's trailing update in between.
+  bool (*try_panel_finish)(int64_t j0, int64_t jb, T* A, int64_t lda, hipStream_t s) = nullptr;
 };
 
 int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, int64_t lda,
