@@ -769,8 +769,8 @@ __global__ void __launch_bounds__(64) tri_inv64_kernel(const float* __restrict__
 // Global memory is read and written in whole blocks, numbered so that
 // consecutive threads touch consecutive addresses in either orientation: the
 // four diagonal blocks are loaded together, and each L block of phase B is
-// loaded into registers while the previous product runs. Only X's lower
-// blocks are written; the caller zeroes the rest.
+// loaded into registers while the previous product runs. The blocks above
+// X's diagonal blocks are written as zeros.
 constexpr int TI_LD = 65;
 constexpr int TI_NT = 512;
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -809,6 +809,15 @@ __global__ void __launch_bounds__(TI_NT) tri_inv256_kernel(const float* __restri
     for (int q = 0; q < SA; ++q) {
       const int f = t + NT * q, b = f >> 12, e = f & 4095;
       Xo[(b * 64 + er(e)) * TI_LD + ec(e)] = v[q];
+    }
+  }
+  // the six blocks (I, J), I < J, of X are zero
+  {
+    constexpr int zI[6] = {0, 0, 0, 1, 1, 2}, zJ[6] = {1, 2, 3, 2, 3, 3};
+#pragma unroll
+    for (int q = 0; q < 6 * SB; ++q) {
+      const int f = t + NT * q, b = f >> 12, e = f & 4095;
+      X[xaddr(zI[b], zJ[b], e)] = 0.0f;
     }
   }
   float pf[SB];   // the next L block of phase B
@@ -901,8 +910,7 @@ __global__ void __launch_bounds__(TI_NT) tri_inv256_kernel(const float* __restri
 }
 
 // X (ld ldx) <- inverse of the 256 x 256 lower triangular L (ld ld), unit or
-// not (trans: of the upper U, through U^T). The strictly upper part of X must
-// already be zero (lower part for trans).
+// not (trans: of the upper U, through U^T); all of X's 256 x 256 is written.
 void tri_inv256(const float* L, int64_t ld, float* X, int64_t ldx, int unit, hipStream_t s, bool trans = false) {
   if (trans) {
     if (unit) hipLaunchKernelGGL((tri_inv256_kernel<true, true>), dim3(1), dim3(TI_NT), 0, s, L, ld, X, ldx);
@@ -1015,10 +1023,7 @@ void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, i
     hipLaunchKernelGGL(rowmap_to_ipiv_kernel, dim3(1), dim3(DB), 0, s, ipiv, base);
     ZH_LAUNCH_CHECK();
   }
-  // U11^-1 (upper) into scr1, whose strictly lower part must be zero
-  hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(DB * DB)), dim3(256), 0, s, (int64_t)DB, mx.scr1,
-                     (int64_t)DB);
-  ZH_LAUNCH_CHECK();
+  // U11^-1 (upper, its lower blocks written as zeros) into scr1
   tri_inv256(P, lda, mx.scr1, DB, 0, s, true);
   ZH_CHECK(hipMemcpy2DAsync(uinv, sizeof(float) * uinv_ld, mx.scr1, sizeof(float) * DB, sizeof(float) * DB, DB,
                             hipMemcpyDeviceToDevice, s));
