@@ -162,11 +162,26 @@ __global__ void __launch_bounds__(256) cols_to_lowp_t_kernel(int64_t cols, int k
   const int64_t j0 = blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
   float mx = 0.0f;
-  for (int k = ty; k < kb; k += 8) {
-    const int64_t j = j0 + tx;
-    const float v = j < cols ? U[k * ldu + j] : 0.0f;
-    tile[k][tx] = v;
-    mx = fmaxf(mx, fabsf(v));
+  // 16 rows per thread in flight: clamped (in-bounds) loads issued together,
+  // then masked
+  const int64_t j = j0 + tx, jc = j < cols ? j : cols - 1;
+  for (int k0 = ty; k0 < kb; k0 += 8 * 16) {
+    float v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = k0 + 8 * q, kc = k < kb ? k : kb - 1;
+      v[q] = U[kc * ldu + jc];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      asm volatile("" : "+v"(v[q]));
+      const int k = k0 + 8 * q;
+      if (k < kb) {
+        const float x = j < cols ? v[q] : 0.0f;
+        tile[k][tx] = x;
+        mx = fmaxf(mx, fabsf(x));
+      }
+    }
   }
   cmax[ty][tx] = mx;
   __syncthreads();
