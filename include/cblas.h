@@ -35,6 +35,24 @@ unsigned int cblas_idamax(const int n, const double *x, const int incx);
 void cblas_dgemv(const CBLAS_ORDER order, const CBLAS_TRANSPOSE transa, const int m, const int n,
                  const double alpha, const double *a, const int lda, const double *x, const int incx,
                  const double beta, double *y, const int incy);
+/* replaces src/cblas.zig:512 (include/cblas.h:208); divides by the diagonal
+ * as trsv.zig does; incx may be negative */
+void cblas_dtrsv(const CBLAS_ORDER order, const CBLAS_UPLO uplo, const CBLAS_TRANSPOSE transa,
+                 const CBLAS_DIAG diag, const int n, const double *a, const int lda, double *x,
+                 const int incx);
+/* replaces src/cblas.zig:523 (include/cblas.h:216): FP32, FP32 MFMA */
+void cblas_sgemm(const CBLAS_ORDER order, const CBLAS_TRANSPOSE transa, const CBLAS_TRANSPOSE transb,
+                 const int m, const int n, const int k, const float alpha, const float *a,
+                 const int lda, const float *b, const int ldb, const float beta, float *c,
+                 const int ldc);
+/* replaces src/cblas.zig:584 (include/cblas.h:235) */
+void cblas_ssyrk(const CBLAS_ORDER order, const CBLAS_UPLO uplo, const CBLAS_TRANSPOSE trans,
+                 const int n, const int k, const float alpha, const float *a, const int lda,
+                 const float beta, float *c, const int ldc);
+/* replaces src/cblas.zig:633 (include/cblas.h:250) */
+void cblas_strsm(const CBLAS_ORDER order, const CBLAS_SIDE side, const CBLAS_UPLO uplo,
+                 const CBLAS_TRANSPOSE transa, const CBLAS_DIAG diag, const int m, const int n,
+                 const float alpha, const float *a, const int lda, float *b, const int ldb);
 /* replaces src/cblas.zig:526 (include/cblas.h:217) */
 void cblas_dgemm(const CBLAS_ORDER order, const CBLAS_TRANSPOSE transa, const CBLAS_TRANSPOSE transb,
                  const int m, const int n, const int k, const double alpha, const double *a,

@@ -17,7 +17,10 @@
 extern "C" {
 #endif
 
-/* replaces the call at src/linalg/lapack.zig:92 (lamch.zig:15-78) */
+/* replaces the call at src/linalg/lapack.zig:92 (lamch.zig:15-78): zml's own
+ * values for f64, i.e. 'N' = 52 (stored mantissa bits), 'M' = -1022 and
+ * 'L' = 1023 (std.math floatExponentMin/Max), where netlib's dlamch gives
+ * 53, -1021 and 1024 */
 double LAPACKE_dlamch(char cmach);
 /* replaces src/linalg/lapack.zig:375 (laswp.zig:12-95) */
 int LAPACKE_dlaswp(int matrix_layout, int n, double *a, int lda, int k1, int k2, const int *ipiv,
@@ -36,6 +39,16 @@ int LAPACKE_dgesv(int matrix_layout, int n, int nrhs, double *a, int lda, int *i
 int LAPACKE_dpotrf2(int matrix_layout, char uplo, int n, double *a, int lda);
 /* replaces src/linalg/lapack.zig:3012 (potrf.zig:17-240) */
 int LAPACKE_dpotrf(int matrix_layout, char uplo, int n, double *a, int lda);
+/* replaces src/linalg/lapack.zig:1160 (getrf.zig:14-152 at T = f32) */
+int LAPACKE_sgetrf(int matrix_layout, int m, int n, float *a, int lda, int *ipiv);
+/* replaces src/linalg/lapack.zig:3004 (potrf.zig:17-240 at T = f32) */
+int LAPACKE_spotrf(int matrix_layout, char uplo, int n, float *a, int lda);
+/* LAPACK dlange: 'M' max |a_ij|, 'O'/'1' largest column sum of |a_ij|, 'I'
+ * largest row sum, 'F'/'E' Frobenius norm; a NaN entry gives NaN. No zml
+ * counterpart (zml has no lange; the residual and backward-error step of a
+ * solve needs ||A||). Returns the norm, or -i (as a double) for an invalid
+ * argument i. */
+double LAPACKE_dlange(int matrix_layout, char norm, int m, int n, const double *a, int lda);
 /* replaces src/linalg/lapack.zig:3466 (potrs.zig:15-117) */
 int LAPACKE_dpotrs(int matrix_layout, char uplo, int n, int nrhs, const double *a, int lda,
                    double *b, int ldb);

@@ -25,6 +25,12 @@ extern "C" {
 #define ZMLHIP_ESYNC (-1002)     /* a bounded device-side wait timed out */
 #define ZMLHIP_ENOTSUP (-1003)   /* shape or option not supported */
 
+/* Calls of one host thread share that thread's device workspaces and internal
+ * streams. Each call therefore starts only after the previous call of the
+ * same thread has finished on the device, whatever stream either was given:
+ * calls queued on two streams by one thread run one after the other. Calls
+ * from different host threads use different workspaces and may overlap. */
+
 /* Library version (major*10000 + minor*100 + patch). */
 int zmlhip_version(void);
 /* Message of the last failing call on this thread ("" if none). */
@@ -66,13 +72,42 @@ int zmlhip_dswap(int n, double *x, int incx, double *y, int incy, void *stream);
 int zmlhip_dgemv(int layout, int transa, int m, int n, double alpha, const double *a, int lda,
                  const double *x, int incx, double beta, double *y, int incy, void *stream);
 
+/* cblas_dtrsv (src/cblas.zig:512; blas/trsv.zig:16-587) */
+int zmlhip_dtrsv(int layout, int uplo, int transa, int diag, int n, const double *a, int lda, double *x,
+                 int incx, void *stream);
+/* FP32 forms: cblas_sgemm (src/cblas.zig:523), cblas_strsm (:633),
+ * cblas_ssyrk (:584); zml's generic gemm/trsm/syrk at T = f32 */
+int zmlhip_sgemm(int layout, int transa, int transb, int m, int n, int k, float alpha, const float *a,
+                 int lda, const float *b, int ldb, float beta, float *c, int ldc, void *stream);
+int zmlhip_strsm(int layout, int side, int uplo, int transa, int diag, int m, int n, float alpha,
+                 const float *a, int lda, float *b, int ldb, void *stream);
+int zmlhip_ssyrk(int layout, int uplo, int trans, int n, int k, float alpha, const float *a, int lda,
+                 float beta, float *c, int ldc, void *stream);
+/* LAPACK dlange norms ('M', 'O'/'1', 'I', 'F'/'E') of an m x n matrix into
+ * *result (device double). No counterpart in zml (it has no lange). */
+int zmlhip_dlange(int layout, char norm, int m, int n, const double *a, int lda, double *result,
+                  void *stream);
+
 /* ---- LAPACK (device pointers; ipiv and info are device memory) -------- */
 /* LAPACKE_dlaswp (src/linalg/lapack.zig:375; lapack/laswp.zig:12-95) */
 int zmlhip_dlaswp(int layout, int n, double *a, int lda, int k1, int k2, const int *ipiv,
                   int incx, void *stream);
-/* LAPACKE_dgetrf (src/linalg/lapack.zig:1169; lapack/getrf.zig:14-152) */
+/* LAPACKE_dgetrf (src/linalg/lapack.zig:1169; lapack/getrf.zig:14-152).
+ * *info: 0, the first zero pivot (1-based), or ZMLHIP_ESYNC when a panel's
+ * bounded cross-workgroup wait gave up (the factors are then invalid; the
+ * condition is cleared at the start of the next factorization). */
 int zmlhip_dgetrf(int layout, int m, int n, double *a, int lda, int *ipiv, int *info,
                   void *stream);
+/* LAPACKE_sgetrf (src/linalg/lapack.zig:1160): the same at T = f32 */
+int zmlhip_sgetrf(int layout, int m, int n, float *a, int lda, int *ipiv, int *info, void *stream);
+/* LAPACKE_spotrf (src/linalg/lapack.zig:3004): potrf at T = f32 */
+int zmlhip_spotrf(int layout, char uplo, int n, float *a, int lda, int *info, void *stream);
+/* Polls a panel's cross-workgroup wait makes before it gives up (default
+ * 2000000; 0 restores the default). A negative value makes every
+ * multi-workgroup panel give up at its first wait, with every kernel still
+ * ending normally: a test hook for the ZMLHIP_ESYNC path. Process-wide.
+ * No counterpart in the reference. */
+int zmlhip_set_panel_spin_limit(long polls);
 /* LAPACKE_dgetrs (src/linalg/lapack.zig:1575; lapack/getrs.zig:13-142) */
 int zmlhip_dgetrs(int layout, char trans, int n, int nrhs, const double *a, int lda,
                   const int *ipiv, double *b, int ldb, void *stream);
@@ -111,7 +146,11 @@ typedef struct zmlhip_ir_result {
 /* A x = b for general A (n x n, device, layout 101/102), nrhs right-hand sides
  * b (ldb), solution into x (ldx, same layout). Synchronous with the host
  * (returns when x is final); work is ordered on `stream`. opts may be NULL.
- * Returns 0, -i for an invalid argument, or a ZMLHIP_E* code. */
+ * Returns 0; k > 0 when the FP64 factorization that a right-hand side fell
+ * back to has an exactly zero pivot at step k (A singular: x is then not a
+ * solution; a column-major A is factored as A^T, so k is that factorization's
+ * step); -i for an invalid argument; or a ZMLHIP_E* code. result->info
+ * reports the low-precision factorization's first zero pivot separately. */
 int zmlhip_dgesv_mixed(int layout, int n, int nrhs, const double *a, int lda, const double *b,
                        int ldb, double *x, int ldx, const zmlhip_ir_opts *opts,
                        zmlhip_ir_result *result, void *stream);
@@ -123,8 +162,12 @@ int zmlhip_dgesv_mixed(int layout, int n, int nrhs, const double *a, int lda, co
  * 16 (classic, GMRES-IR, FP64 Cholesky as in zmlhip_dgesv_mixed). result->info
  * is the first non-positive pivot of the unshifted low-precision
  * factorization (0 if none); a failure there is answered by factoring again
- * with a small diagonal shift. Synchronous with the host. Returns 0, -i for an
- * invalid argument, or a ZMLHIP_E* code. */
+ * with a small diagonal shift. Synchronous with the host. Returns 0; k > 0
+ * when A is not positive definite (LAPACK's dposv info: the unshifted
+ * low-precision factorization's first non-positive pivot k when four
+ * shifted attempts fail too, x untouched; or that of the FP64 Cholesky a
+ * right-hand side fell back to); -i for an invalid argument; or a ZMLHIP_E*
+ * code. */
 int zmlhip_dposv_mixed(int layout, char uplo, int n, int nrhs, const double *a, int lda,
                        const double *b, int ldb, double *x, int ldx, const zmlhip_ir_opts *opts,
                        zmlhip_ir_result *result, void *stream);

@@ -44,6 +44,7 @@ def lib() -> ctypes.CDLL:
             raise RuntimeError(f"oracle library missing: build it with `make -C {HERE}`")
         L = ctypes.CDLL(LIB_PATH)
         d, i, p, u64 = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64
+        f = ctypes.c_float
         sig = {
             "zr_lamch": (d, [ctypes.c_char]),
             "zr_iamax": (i, [i, p, i]),
@@ -53,6 +54,19 @@ def lib() -> ctypes.CDLL:
             "zr_trsm": (i, [i, i, i, i, i, i, i, d, p, i, p, i]),
             "zr_syrk": (i, [i, i, i, i, i, d, p, i, d, p, i]),
             "zr_gemv": (i, [i, i, i, i, d, p, i, p, i, d, p, i]),
+            "zr_trsv": (i, [i, i, i, i, i, p, i, p, i]),
+            "zr_lange": (d, [i, ctypes.c_char, i, i, p, i]),
+            "zr_slamch": (ctypes.c_float, [ctypes.c_char]),
+            "zr_sgemm": (i, [i, i, i, i, i, i, f, p, i, p, i, f, p, i]),
+            "zr_strsm": (i, [i, i, i, i, i, i, i, f, p, i, p, i]),
+            "zr_ssyrk": (i, [i, i, i, i, i, f, p, i, f, p, i]),
+            "zr_strsv": (i, [i, i, i, i, i, p, i, p, i]),
+            "zr_sgetrf": (i, [i, i, i, p, i, p]),
+            "zr_sgetrs": (i, [i, i, i, i, p, i, p, p, i]),
+            "zr_spotrf": (i, [i, i, i, p, i]),
+            "zr_spotrs": (i, [i, i, i, i, p, i, p, i]),
+            "zr_sgetrf_unblocked": (i, [i, i, i, p, i, p]),
+            "zr_spotrf_lower_unblocked": (i, [i, p, i]),
             "zr_set_trsm_faithful_lda": (None, [i]),
             "zr_laswp": (i, [i, i, p, i, i, i, p, i]),
             "zr_getrf2": (i, [i, i, i, p, i, p]),
@@ -124,6 +138,71 @@ def syrk(order, uplo, trans, n, k, alpha, a, lda, beta, c, ldc):
 
 def gemv(order, trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
     return lib().zr_gemv(order, trans, m, n, alpha, _p(a), lda, _p(x), incx, beta, _p(y), incy)
+
+
+def trsv(order, uplo, trans, diag, n, a, lda, x, incx):
+    return lib().zr_trsv(order, uplo, trans, diag, n, _p(a), lda, _p(x), incx)
+
+
+def lamch(c):
+    return lib().zr_lamch(c.encode() if isinstance(c, str) else c)
+
+
+def lange(order, norm, m, n, a, ld):
+    return lib().zr_lange(order, norm.encode() if isinstance(norm, str) else norm, m, n, _p(a), ld)
+
+
+# ---- f32 (zml's generic routines at T = f32) ----------------------------------
+def _f32(a):
+    a = np.asarray(a)
+    if a.dtype != np.float32:
+        raise TypeError("the f32 oracle takes float32 buffers")
+    return a
+
+
+def slamch(c):
+    return lib().zr_slamch(c.encode() if isinstance(c, str) else c)
+
+
+def sgemm(order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    return lib().zr_sgemm(order, ta, tb, m, n, k, alpha, _p(_f32(a)), lda, _p(_f32(b)), ldb, beta,
+                          _p(_f32(c)), ldc)
+
+
+def strsm(order, side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb):
+    return lib().zr_strsm(order, side, uplo, ta, diag, m, n, alpha, _p(_f32(a)), lda, _p(_f32(b)), ldb)
+
+
+def ssyrk(order, uplo, trans, n, k, alpha, a, lda, beta, c, ldc):
+    return lib().zr_ssyrk(order, uplo, trans, n, k, alpha, _p(_f32(a)), lda, beta, _p(_f32(c)), ldc)
+
+
+def strsv(order, uplo, trans, diag, n, a, lda, x, incx):
+    return lib().zr_strsv(order, uplo, trans, diag, n, _p(_f32(a)), lda, _p(_f32(x)), incx)
+
+
+def sgetrf(order, m, n, a, lda, ipiv):
+    return lib().zr_sgetrf(order, m, n, _p(_f32(a)), lda, _p(ipiv))
+
+
+def sgetrs(order, trans, n, nrhs, a, lda, ipiv, b, ldb):
+    return lib().zr_sgetrs(order, trans, n, nrhs, _p(_f32(a)), lda, _p(ipiv), _p(_f32(b)), ldb)
+
+
+def spotrf(order, uplo, n, a, lda):
+    return lib().zr_spotrf(order, uplo, n, _p(_f32(a)), lda)
+
+
+def spotrs(order, uplo, n, nrhs, a, lda, b, ldb):
+    return lib().zr_spotrs(order, uplo, n, nrhs, _p(_f32(a)), lda, _p(_f32(b)), ldb)
+
+
+def sgetrf_unblocked(order, m, n, a, lda, ipiv):
+    return lib().zr_sgetrf_unblocked(order, m, n, _p(_f32(a)), lda, _p(ipiv))
+
+
+def spotrf_lower_unblocked(n, a, lda):
+    return lib().zr_spotrf_lower_unblocked(n, _p(_f32(a)), lda)
 
 
 def set_trsm_faithful_lda(on: bool):

@@ -34,7 +34,8 @@ enum { ZR_LEFT = 141, ZR_RIGHT = 142 };
 #define ZR_EINVAL (-1)
 
 /* ---- machine constants and block sizes ---------------------------------- */
-double zr_lamch(char cmach);            /* lapack/lamch.zig:15-78            */
+double zr_lamch(char cmach);            /* lapack/lamch.zig:15-78: zml's values,
+                                           N = 52, M = -1022, L = 1023 for f64 */
 int zr_ilaenv_nb(const char* name);     /* lapack/ilaenv.zig:143-148,227-233 */
 
 /* ---- BLAS (src/linalg/blas/) --------------------------------------- */
@@ -50,6 +51,9 @@ int zr_syrk(int order, int uplo, int trans, int n, int k, double alpha,
             const double* a, int lda, double beta, double* c, int ldc); /* syrk.zig */
 int zr_gemv(int order, int trans, int m, int n, double alpha, const double* a, int lda,
             const double* x, int incx, double beta, double* y, int incy); /* gemv.zig */
+
+int zr_trsv(int order, int uplo, int trans, int diag, int n, const double* a, int lda,
+            double* x, int incx);                               /* trsv.zig */
 
 /* trsm.zig:137-142,221-229 divide b[k + j*lda] (not ldb) by the diagonal in
  * the left/no-transpose/non-unit branches. 1 (default) restates that read
@@ -67,6 +71,34 @@ int zr_potrf2(int order, int uplo, int n, double* a, int lda);
 int zr_potrf(int order, int uplo, int n, double* a, int lda);
 int zr_potrs(int order, int uplo, int n, int nrhs, const double* a, int lda, double* b, int ldb);
 int zr_posv(int order, int uplo, int n, int nrhs, double* a, int lda, double* b, int ldb);
+
+/* ---- f32: the same routines at zml's T = f32 (zr_s*, one generic source) -- */
+float zr_slamch(char cmach);
+int zr_siamax(int n, const float* x, int incx);
+int zr_sscal(int n, float alpha, float* x, int incx);
+int zr_sswap(int n, float* x, int incx, float* y, int incy);
+int zr_sgemm(int order, int transa, int transb, int m, int n, int k, float alpha,
+             const float* a, int lda, const float* b, int ldb, float beta, float* c, int ldc);
+int zr_strsm(int order, int side, int uplo, int transa, int diag, int m, int n,
+             float alpha, const float* a, int lda, float* b, int ldb);
+int zr_ssyrk(int order, int uplo, int trans, int n, int k, float alpha,
+             const float* a, int lda, float beta, float* c, int ldc);
+int zr_sgemv(int order, int trans, int m, int n, float alpha, const float* a, int lda,
+             const float* x, int incx, float beta, float* y, int incy);
+int zr_strsv(int order, int uplo, int trans, int diag, int n, const float* a, int lda,
+             float* x, int incx);
+int zr_slaswp(int order, int n, float* a, int lda, int k1, int k2, const int32_t* ipiv, int incx);
+int zr_sgetrf2(int order, int m, int n, float* a, int lda, int32_t* ipiv);
+int zr_sgetrf(int order, int m, int n, float* a, int lda, int32_t* ipiv);
+int zr_sgetrs(int order, int trans, int n, int nrhs, const float* a, int lda,
+              const int32_t* ipiv, float* b, int ldb);
+int zr_sgesv(int order, int n, int nrhs, float* a, int lda, int32_t* ipiv, float* b, int ldb);
+int zr_spotrf2(int order, int uplo, int n, float* a, int lda);
+int zr_spotrf(int order, int uplo, int n, float* a, int lda);
+int zr_spotrs(int order, int uplo, int n, int nrhs, const float* a, int lda, float* b, int ldb);
+int zr_sposv(int order, int uplo, int n, int nrhs, float* a, int lda, float* b, int ldb);
+int zr_sgetrf_unblocked(int order, int m, int n, float* a, int lda, int32_t* ipiv);
+int zr_spotrf_lower_unblocked(int n, float* a, int lda);
 
 /* ---- closed forms (derived, not zml files) ------------------------------ */
 /* Unblocked right-looking LU with zml's pivot, scaling and update rules:
@@ -86,6 +118,7 @@ double zr_rng_uniform(uint64_t seed, uint64_t ctr);      /* [0, 1), 53-bit */
  * lo + (hi - lo) * u, stored in the given layout with leading dimension ld. */
 void zr_fill_uniform(int order, int m, int n, double* a, int ld, uint64_t seed, double lo, double hi);
 /* ||A||_inf (max row sum of |a_ij|) of an m x n matrix. */
+double zr_lange(int order, char norm, int m, int n, const double* a, int ld); /* no zml counterpart */
 double zr_norm_inf(int order, int m, int n, const double* a, int ld);
 /* r = b - A x in FP64 (row dots, ascending j), returns ||r||_inf. */
 double zr_residual_inf(int order, int n, const double* a, int lda, const double* x, const double* b, double* r);

@@ -10,6 +10,8 @@ It reads the real-valued (f64) part of
     test/linalg/blas/gemm.zig:6-260   (8 chained calls, exact)
     test/linalg/blas/trsm.zig:6-892   (32 chained calls, rel 1e-7)
     test/linalg/blas/syrk.zig:6-287   (8 chained calls, exact)
+    test/linalg/blas/gemv.zig:6-320   (8 calls, +-2 strides, exact)
+    test/linalg/blas/trsv.zig:6-420   (16 calls, +-2 strides, rel 1e-7)
 
 and records, in order, every buffer initialisation, every call with its
 argument tokens and every expected element. The loop-generated vectors of
@@ -24,7 +26,7 @@ import os
 import re
 import sys
 
-CALL_RE = re.compile(r"^\s*(gemm|trsm|syrk)\((.*)\)\s*catch unreachable;")
+CALL_RE = re.compile(r"^\s*(gemm|trsm|syrk|gemv|trsv)\((.*)\)\s*catch unreachable;")
 MEMCPY_RE = re.compile(r"^\s*@memcpy\((\w+)\.ptr,\s*&\[_\]f64\{")
 EXPECT_RE = re.compile(
     r"expect(Equal|ApproxEqRel)\(([-0-9.eE+]+),\s*(\w+)\[(\d+)\](?:,\s*([-0-9.eE+]+))?\)")
@@ -118,14 +120,15 @@ def vector_fixtures():
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     base = os.path.join(ref, "test", "linalg", "blas")
-    out = {name: parse_file(os.path.join(base, name + ".zig")) for name in ("gemm", "trsm", "syrk")}
+    names = ("gemm", "trsm", "syrk", "gemv", "trsv")
+    out = {name: parse_file(os.path.join(base, name + ".zig")) for name in names}
     for name, fx in out.items():
         fx["source"] = f"test/linalg/blas/{name}.zig"
     out.update(vector_fixtures())
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "blas_fixtures.json")
     with open(dst, "w") as f:
         json.dump(out, f, separators=(",", ":"))
-    for name in ("gemm", "trsm", "syrk"):
+    for name in names:
         ev = out[name]["events"]
         print(name, "calls", sum(e["op"] == "call" for e in ev), "expects", sum(e["op"] == "expect" for e in ev))
 

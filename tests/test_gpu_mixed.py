@@ -171,14 +171,44 @@ def test_mixed_gmres_and_fp64_fallbacks(zh, torch_cuda, order):
     n = 600
     A = config3_matrix(n, 0.01, 11)
     B = A @ np.ones((n, 1))
-    # no classic refinement allowed: GMRES-IR must finish the job
-    X, res = run_mixed(zh, torch_cuda, A, B, order, 2, max_iter=0)
+    # one classic refinement step only (BF16 needs more): GMRES-IR must finish the job
+    X, res = run_mixed(zh, torch_cuda, A, B, order, 2, max_iter=1)
+    assert res.iterations <= 1
     assert res.fallback == 1 and res.gmres_iterations > 0
     assert backward_error(A, X, B) <= 16
     # neither: the FP64 factorization solves it
-    X, res = run_mixed(zh, torch_cuda, A, B, order, 2, max_iter=0, no_gmres=1)
+    X, res = run_mixed(zh, torch_cuda, A, B, order, 2, max_iter=1, no_gmres=1)
+    assert res.iterations <= 1
     assert res.fallback == 2
     assert backward_error(A, X, B) <= 16
+
+
+@pytest.mark.parametrize("order", [ROW, COL])
+def test_mixed_lu_singular_matrix_is_reported(zh, torch_cuda, order):
+    """An exactly singular A (zero column 100): the low-precision factor and
+    the FP64 fallback both find the zero pivot; the call returns its column
+    (LAPACK's info > 0) and the backward error is not a small number."""
+    from zml_amd._lib import IROpts, IRResult
+    torch = torch_cuda
+    n = 600
+    A = config3_matrix(n, 0.1, 31, scaled=False)
+    A[:, 100] = 0.0
+    B = A @ np.ones((n, 1))
+    a = torch.from_numpy(store(order, A, n)).cuda()
+    ldb = n if order == COL else 1
+    b = torch.from_numpy(store(order, B, ldb)).cuda()
+    x = torch.zeros_like(b)
+    r = IRResult()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = zh.zmlhip_dgesv_mixed(order, n, 1, P(a), n, P(b), ldb, P(x), ldb,
+                               ctypes.byref(IROpts(precision=1, max_iter=3)), ctypes.byref(r), s)
+    torch.cuda.synchronize()
+    print(f"rc {rc}, info {r.info}, fallback {r.fallback}, be {r.backward_error}")
+    assert r.info == 101 and r.fallback == 2
+    # a column-major A reaches the FP64 factorization as A^T (rows contiguous),
+    # whose zero pivot (the zero row) shows at the last step
+    assert rc == (101 if order == ROW else n)
+    assert not (r.backward_error <= 16)   # NaN or large, never a small number
 
 
 def test_mixed_rejects_bad_arguments(zh, torch_cuda):

@@ -167,6 +167,30 @@ def test_mixed_cholesky_indefinite_low_precision_gets_shifted(zh, torch_cuda):
     assert backward_errors(A, X, B).max() <= 16
 
 
+@pytest.mark.parametrize("order", [ROW, COL])
+def test_mixed_cholesky_not_positive_definite_is_reported(zh, torch_cuda, order):
+    """A clearly indefinite A: the shifted low-precision factorizations keep
+    failing, and the call returns the first non-positive pivot (LAPACK's info)
+    with x untouched, as dposv would."""
+    from zml_amd._lib import IROpts, IRResult
+    torch = torch_cuda
+    n = 300
+    A = np.eye(n)
+    A[10, 10] = -100.0
+    B = np.ones((n, 1))
+    a = torch.from_numpy(store(order, A, n)).cuda()
+    ldb = n if order == COL else 1
+    b = torch.from_numpy(store(order, B, ldb)).cuda()
+    x = torch.full_like(b, 7.0)
+    r = IRResult()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = zh.zmlhip_dposv_mixed(order, b"L", n, 1, P(a), n, P(b), ldb, P(x), ldb, ctypes.byref(IROpts(precision=1)),
+                               ctypes.byref(r), s)
+    torch.cuda.synchronize()
+    assert rc == r.info == 11
+    assert np.all(x.cpu().numpy() == 7.0)
+
+
 def test_mixed_cholesky_rejects_bad_arguments(zh, torch_cuda):
     from zml_amd._lib import IRResult
     r = IRResult()

@@ -336,3 +336,178 @@ def test_backward_error_metric(oracle_lib):
     x2 = x.copy()
     x2[0] += 1e-6
     assert O.backward_error(O.ROW, n, a, n, x2, b) > 1e3
+
+
+# ---- gemv / trsv: the reference's vectors (test/linalg/blas/gemv.zig, trsv.zig)
+class OracleBackendL2(OracleBackend):
+    gemv = staticmethod(O.gemv)
+    trsv = staticmethod(O.trsv)
+
+
+@pytest.mark.parametrize("name", ["gemv", "trsv"])
+def test_reference_level2_golden_vectors(oracle_lib, name):
+    fails = fixture_replay.replay(name, OracleBackendL2)
+    assert not fails, "\n".join(fails[:10])
+
+
+def test_planted_trsv_mistake_is_caught(oracle_lib):
+    """A trsv that ignored the unit-diagonal flag fails the reference's vectors."""
+    class NoUnit(OracleBackendL2):
+        @staticmethod
+        def trsv(order, uplo, trans, diag, *rest):
+            return O.trsv(order, uplo, trans, 131, *rest)
+    assert fixture_replay.replay("trsv", NoUnit)
+
+
+@pytest.mark.parametrize("order", [O.COL, O.ROW])
+@pytest.mark.parametrize("uplo", [O.UPPER, O.LOWER])
+@pytest.mark.parametrize("trans", [O.NOTRANS, O.TRANS])
+@pytest.mark.parametrize("diag", [O.NONUNIT, O.UNIT])
+@pytest.mark.parametrize("incx", [1, -3])
+def test_trsv_solves_the_triangular_system(oracle_lib, order, uplo, trans, diag, incx):
+    """x = op(T)^-1 b against numpy's solve of the dense triangle."""
+    n = 37
+    rng = np.random.default_rng(n + incx + order + uplo + trans + diag)
+    M = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
+    T = np.triu(M) if uplo == O.UPPER else np.tril(M)
+    if diag == O.UNIT:
+        np.fill_diagonal(T, 1.0)
+    a = (T.T if order == O.COL else T).copy().ravel()
+    if diag == O.UNIT:   # the stored diagonal must not be read
+        av = a.reshape(n, n)
+        np.fill_diagonal(av, 7.0)
+    b = rng.uniform(-1, 1, n)
+    x = np.zeros(abs(incx) * (n - 1) + 1)
+    x[::abs(incx)] = b if incx > 0 else b[::-1]
+    assert O.trsv(order, uplo, trans, diag, n, a, n, x, incx) == 0
+    got = x[::abs(incx)] if incx > 0 else x[::abs(incx)][::-1]
+    want = np.linalg.solve(T if trans == O.NOTRANS else T.T, b)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-13)
+    assert O.trsv(order, uplo, trans, diag, n, a, n, x, 0) == -1   # trsv.zig:76
+
+
+# ---- lamch: zml's values are IEEE facts (lamch.zig:15-78 over std.math)
+@pytest.mark.parametrize("dt,fn", [(np.float64, "lamch"), (np.float32, "slamch")])
+def test_lamch_is_pinned_by_ieee_constants(oracle_lib, dt, fn):
+    fi = np.finfo(dt)
+    f = getattr(O, fn)
+    assert f("E") == fi.eps / 2            # rnd = 1: eps = floatEps / 2
+    assert f("P") == fi.eps
+    assert f("S") == fi.tiny                # 1/max < tiny, so sfmin = floatMin
+    assert f("S") == (2.0 ** -1022 if dt == np.float64 else 2.0 ** -126)
+    assert 1.0 / fi.max < fi.tiny
+    assert f("U") == fi.tiny
+    assert f("O") == fi.max
+    assert f("B") == 2 and f("R") == 1
+    assert f("N") == fi.nmant               # floatMantissaBits: stored bits (52 / 23)
+    assert f("M") == fi.minexp              # floatExponentMin (-1022 / -126)
+    assert f("L") == fi.maxexp - 1          # floatExponentMax (1023 / 127)
+    assert f("e") == f("E") and f("x") == 0
+
+
+# ---- the |p| < sfmin branch of getrf2 (getrf2.zig:77-87): divide, not reciprocal
+@pytest.mark.parametrize("order", [O.COL, O.ROW])
+def test_subnormal_pivot_divides(oracle_lib, order):
+    """With p = 2^-1050 the reciprocal overflows (1/p = 2^1050 > max), so only
+    the division branch yields the finite multiplier 2^-1060 / p = 2^-10."""
+    p, q = 2.0 ** -1050, 2.0 ** -1060
+    A = np.array([[p, 1.0, 2.0], [q, 5.0, 0.5], [0.0, 3.0, 1.0]])
+    assert p < O.lamch("S") and np.isinf(1.0 / p)
+    for fn in (O.getrf, O.getrf2, O.getrf_unblocked):
+        a = (A.T if order == O.COL else A).copy().ravel()
+        ip = np.zeros(3, np.int32)
+        assert fn(order, 3, 3, a, 3, ip) == 0
+        LU = a.reshape(3, 3).T if order == O.COL else a.reshape(3, 3)
+        assert list(ip) == [1, 2, 3]
+        assert LU[1, 0] == 2.0 ** -10          # exact: q / p
+        assert np.all(np.isfinite(LU))
+    # a planted mistake: the reciprocal branch at this pivot gives an infinite L
+    assert np.isinf(q * (1.0 / p))
+
+
+def test_getrf_info_and_subnormal_singular_column(oracle_lib):
+    """A column of subnormals is not singular (p != 0); an exact zero column is."""
+    n = 4
+    A = np.eye(n)
+    A[:, 1] = [0.0, 2.0 ** -1070, 2.0 ** -1072, 0.0]
+    a = A.copy().ravel()
+    ip = np.zeros(n, np.int32)
+    assert O.getrf(O.ROW, n, n, a, n, ip) == 0
+    B = np.eye(n)
+    B[:, 2] = 0.0
+    b = B.copy().ravel()
+    assert O.getrf(O.ROW, n, n, b, n, np.zeros(n, np.int32)) == 3
+
+
+# ---- lange (no zml counterpart): numpy's norms
+@pytest.mark.parametrize("order", [O.COL, O.ROW])
+@pytest.mark.parametrize("m,n", [(1, 1), (7, 3), (40, 65)])
+def test_lange_matches_numpy(oracle_lib, order, m, n):
+    A = np.random.default_rng(m * n).uniform(-3, 3, (m, n))
+    a = (A.T if order == O.COL else A).copy().ravel()
+    ld = m if order == O.COL else n
+    assert O.lange(order, "M", m, n, a, ld) == np.abs(A).max()
+    assert O.lange(order, "I", m, n, a, ld) == pytest.approx(np.linalg.norm(A, np.inf), rel=1e-15)
+    assert O.lange(order, "O", m, n, a, ld) == pytest.approx(np.linalg.norm(A, 1), rel=1e-15)
+    assert O.lange(order, "1", m, n, a, ld) == O.lange(order, "O", m, n, a, ld)
+    assert O.lange(order, "F", m, n, a, ld) == pytest.approx(np.linalg.norm(A, "fro"), rel=1e-14)
+    a[0] = np.nan
+    assert np.isnan(O.lange(order, "M", m, n, a, ld)) and np.isnan(O.lange(order, "I", m, n, a, ld))
+    assert O.lange(order, "M", 0, n, a, ld) == 0.0
+
+
+# ---- f32 instantiation: the same generic source at T = f32
+class OracleBackendS:
+    gemm = staticmethod(O.sgemm)
+    trsm = staticmethod(O.strsm)
+    syrk = staticmethod(O.ssyrk)
+
+
+@pytest.mark.parametrize("name", ["gemm", "syrk"])
+def test_f32_reference_vectors_exact(oracle_lib, name):
+    """The integer-valued gemm/syrk vectors are exact in f32 as well."""
+    fails = fixture_replay.replay(name, OracleBackendS, dtype=np.float32)
+    assert not fails, "\n".join(fails[:10])
+
+
+def test_f32_trsm_reference_vectors(oracle_lib):
+    """The chained f64 vectors hold at f32 precision: rel 1e-4 (the worst chain
+    amplifies the f32 rounding to ~4e-5)."""
+    fails = fixture_replay.replay("trsm", OracleBackendS, dtype=np.float32, rel_floor=1e-4)
+    assert not fails, "\n".join(fails[:10])
+
+
+@pytest.mark.parametrize("order", [O.COL, O.ROW])
+def test_f32_getrf_closed_form_bitwise(oracle_lib, order):
+    n = 150
+    a = O.fill_uniform(order, n, n, 11).astype(np.float32)
+    b = a.copy()
+    ipa, ipb = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    assert O.sgetrf(order, n, n, a, n, ipa) == O.sgetrf_unblocked(order, n, n, b, n, ipb) == 0
+    assert np.array_equal(ipa, ipb) and np.array_equal(a, b)
+
+
+def test_f32_potrf_closed_form_bitwise(oracle_lib):
+    n = 130
+    B = np.random.default_rng(3).standard_normal((n, n))
+    S = (B @ B.T / n + np.eye(n)).astype(np.float32)
+    a = S.T.copy().ravel()
+    b = a.copy()
+    assert O.spotrf(O.COL, O.LOWER, n, a, n) == O.spotrf_lower_unblocked(n, b, n) == 0
+    L, L2 = np.tril(a.reshape(n, n).T), np.tril(b.reshape(n, n).T)
+    assert np.array_equal(L, L2)
+    assert np.abs(L.astype(np.float64) @ L.T.astype(np.float64) - S).max() < 1e-5
+
+
+def test_f32_lu_of_integer_matrix_is_exact(oracle_lib):
+    """L = [[1,0,0],[2,1,0],[3,4,1]], U = [[2,1,1],[0,3,1],[0,0,4]]: every
+    intermediate of A = L U is an integer, so the f32 factors are exact."""
+    L = np.array([[1, 0, 0], [0.5, 1, 0], [0.25, 0.5, 1]])
+    U = np.array([[8, 1, 1], [0, 6, 1], [0, 0, 4]])
+    A = (L @ U).astype(np.float32)
+    a = A.copy().ravel()
+    ip = np.zeros(3, np.int32)
+    assert O.sgetrf(O.ROW, 3, 3, a, 3, ip) == 0
+    assert list(ip) == [1, 2, 3]
+    F = a.reshape(3, 3)
+    assert np.array_equal(np.tril(F, -1) + np.eye(3), L) and np.array_equal(np.triu(F), U)

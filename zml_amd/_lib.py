@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libzmlhip.so")
 
 _c_int, _c_double, _p, _c_char = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_char
+_c_float, _c_long = ctypes.c_float, ctypes.c_long
 
 # name -> (restype, argtypes); mirrors include/cblas.h, clapack.h, zmlhip.h
 SIGNATURES = {
@@ -27,6 +28,13 @@ SIGNATURES = {
                            _c_double, _p, _c_int]),
     "cblas_dtrsm": (None, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _p,
                            _c_int, _p, _c_int]),
+    "cblas_dtrsv": (None, [_c_int, _c_int, _c_int, _c_int, _c_int, _p, _c_int, _p, _c_int]),
+    "cblas_sgemm": (None, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_float, _p, _c_int,
+                           _p, _c_int, _c_float, _p, _c_int]),
+    "cblas_ssyrk": (None, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_float, _p, _c_int,
+                           _c_float, _p, _c_int]),
+    "cblas_strsm": (None, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_float, _p,
+                           _c_int, _p, _c_int]),
     # clapack.h
     "LAPACKE_dlamch": (_c_double, [_c_char]),
     "LAPACKE_dlaswp": (_c_int, [_c_int, _c_int, _p, _c_int, _c_int, _c_int, _p, _c_int]),
@@ -38,6 +46,9 @@ SIGNATURES = {
     "LAPACKE_dpotrf": (_c_int, [_c_int, _c_char, _c_int, _p, _c_int]),
     "LAPACKE_dpotrs": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _c_int]),
     "LAPACKE_dposv": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _c_int]),
+    "LAPACKE_sgetrf": (_c_int, [_c_int, _c_int, _c_int, _p, _c_int, _p]),
+    "LAPACKE_spotrf": (_c_int, [_c_int, _c_char, _c_int, _p, _c_int]),
+    "LAPACKE_dlange": (_c_double, [_c_int, _c_char, _c_int, _c_int, _p, _c_int]),
     # zmlhip.h
     "zmlhip_version": (_c_int, []),
     "zmlhip_last_error": (ctypes.c_char_p, []),
@@ -55,6 +66,17 @@ SIGNATURES = {
     "zmlhip_dswap": (_c_int, [_c_int, _p, _c_int, _p, _c_int, _p]),
     "zmlhip_dgemv": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_double, _p, _c_int, _p, _c_int,
                               _c_double, _p, _c_int, _p]),
+    "zmlhip_dtrsv": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _p, _c_int, _p, _c_int, _p]),
+    "zmlhip_sgemm": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_float, _p, _c_int,
+                              _p, _c_int, _c_float, _p, _c_int, _p]),
+    "zmlhip_strsm": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_float, _p,
+                              _c_int, _p, _c_int, _p]),
+    "zmlhip_ssyrk": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_float, _p, _c_int,
+                              _c_float, _p, _c_int, _p]),
+    "zmlhip_dlange": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _p]),
+    "zmlhip_sgetrf": (_c_int, [_c_int, _c_int, _c_int, _p, _c_int, _p, _p, _p]),
+    "zmlhip_spotrf": (_c_int, [_c_int, _c_char, _c_int, _p, _c_int, _p, _p]),
+    "zmlhip_set_panel_spin_limit": (_c_int, [_c_long]),
     "zmlhip_dlaswp": (_c_int, [_c_int, _c_int, _p, _c_int, _c_int, _c_int, _p, _c_int, _p]),
     "zmlhip_dgetrf": (_c_int, [_c_int, _c_int, _c_int, _p, _c_int, _p, _p, _p]),
     "zmlhip_dgetrs": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _p, _c_int, _p]),

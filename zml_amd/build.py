@@ -1,6 +1,7 @@
 """Builds libzmlhip.so (gfx950) in-tree with hipcc.
 
     python -m zml_amd.build            # compile every source, link the library
+    python -m zml_amd.build --changed  # recompile only sources newer than their objects
 
 Objects go to zml_amd/_build/, the library to zml_amd/libzmlhip.so. Every
 HIP translation unit is compiled with packed FP32 instructions disabled
@@ -44,12 +45,24 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = True) -> str:
+def _stale(src: str) -> bool:
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    deps += [os.path.join(HERE, "..", "include", f) for f in os.listdir(os.path.join(HERE, "..", "include"))]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(jobs: int | None = None, verbose: bool = True, changed_only: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
-    jobs = jobs or min(8, len(srcs), os.cpu_count() or 4)
+    todo = [x for x in srcs if _stale(x)] if changed_only else srcs
+    jobs = jobs or max(1, min(8, len(todo), os.cpu_count() or 4))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [os.path.join(BUILD, os.path.basename(x) + ".o") for x in srcs]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -57,10 +70,10 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources")
+        print(f"built {LIB} from {len(srcs)} sources ({len(todo)} compiled)")
     return LIB
 
 
 if __name__ == "__main__":
-    build()
+    build(changed_only="--changed" in sys.argv[1:])
     sys.exit(0)

@@ -1,5 +1,6 @@
-// FP64 Level-1/3 helpers on the device: TRSM, SYRK, IDAMAX, DSCAL, DSWAP,
-// row interchanges (LASWP) and transposes.
+// Level-1/2/3 helpers on the device: TRSM and SYRK (FP64 and FP32), IDAMAX,
+// DSCAL, DSWAP, row interchanges (LASWP), transposes, GEMV, TRSV and the
+// matrix norms of LANGE.
 //
 // TRSM is blocked: 32 x 32 diagonal blocks are solved by substitution (one
 // thread per right-hand-side vector, the block staged in LDS), and the rest
@@ -18,28 +19,32 @@ namespace {
 constexpr int TB = 32;   // diagonal block size of the blocked TRSM
 
 // Solves M x = b for each vector x of a 32-row block (left side: vectors are
-// columns of B; right side: rows of B, with M the transpose of op(T)).
+// columns of B; right side: rows of B, with M the transpose of op(T)), in the
+// column-sweep order of zml's no-transpose loops.
 // forward: M lower (k ascending); else M upper (k descending).
-template <bool LEFT>
+template <typename T, bool LEFT>
 __global__ void __launch_bounds__(256)
-trsm_diag_kernel(int nb, const double* __restrict__ a, int64_t lda, int need_t, int forward,
-                 int unit, double* __restrict__ b, int64_t ldb, int64_t nvec) {
-  __shared__ double M[TB][TB + 1];
-  __shared__ double dinv[TB];
+trsm_diag_kernel(int nb, const T* __restrict__ a, int64_t lda, int need_t, int forward,
+                 int unit, T* __restrict__ b, int64_t ldb, int64_t nvec) {
+  // separately rounded multiply and subtract, as zml's substitution loops
+  // (trsm.zig, trsv.zig): exact cancellations stay exact
+#pragma clang fp contract(off)
+  __shared__ T M[TB][TB + 1];
+  __shared__ T dinv[TB];
   for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
     const int r = e % TB, c = e / TB;
-    double v = 0.0;
+    T v = T(0);
     if (r < nb && c < nb) v = need_t ? a[c + (int64_t)r * lda] : a[r + (int64_t)c * lda];
     M[r][c] = v;
   }
   __syncthreads();
-  if (threadIdx.x < TB) dinv[threadIdx.x] = threadIdx.x < nb ? 1.0 / M[threadIdx.x][threadIdx.x] : 1.0;
+  if (threadIdx.x < TB) dinv[threadIdx.x] = threadIdx.x < nb ? T(1) / M[threadIdx.x][threadIdx.x] : T(1);
   __syncthreads();
   const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (v >= nvec) return;
-  double x[TB];
+  T x[TB];
 #pragma unroll
-  for (int r = 0; r < TB; ++r) x[r] = r < nb ? (LEFT ? b[r + v * ldb] : b[v + (int64_t)r * ldb]) : 0.0;
+  for (int r = 0; r < TB; ++r) x[r] = r < nb ? (LEFT ? b[r + v * ldb] : b[v + (int64_t)r * ldb]) : T(0);
   if (forward) {
 #pragma unroll
     for (int k = 0; k < TB; ++k) {
@@ -175,8 +180,9 @@ __global__ void laswp_cm_kernel(int64_t ncols, double* __restrict__ a, int64_t l
 
 constexpr int TT = 32;   // transpose tile
 
-__global__ void transpose_square_kernel(int64_t n, double* __restrict__ a, int64_t ld) {
-  __shared__ double t1[TT][TT + 1], t2[TT][TT + 1];
+template <typename T>
+__global__ void transpose_square_kernel(int64_t n, T* __restrict__ a, int64_t ld) {
+  __shared__ T t1[TT][TT + 1], t2[TT][TT + 1];
   const int64_t I = blockIdx.y, J = blockIdx.x;
   if (I > J) return;
   const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
@@ -199,9 +205,10 @@ __global__ void transpose_square_kernel(int64_t n, double* __restrict__ a, int64
 }
 
 // dst[c * ldd + r] = src[r * lds + c] for r < rows, c < cols.
-__global__ void transpose_copy_kernel(int64_t rows, int64_t cols, const double* __restrict__ src,
-                                      int64_t lds, double* __restrict__ dst, int64_t ldd) {
-  __shared__ double t[TT][TT + 1];
+template <typename T>
+__global__ void transpose_copy_kernel(int64_t rows, int64_t cols, const T* __restrict__ src,
+                                      int64_t lds, T* __restrict__ dst, int64_t ldd) {
+  __shared__ T t[TT][TT + 1];
   const int64_t R = blockIdx.y, Cb = blockIdx.x;
   const int tx = threadIdx.x, ty = threadIdx.y;
   for (int r = ty; r < TT; r += 8) {
@@ -222,11 +229,12 @@ inline int grid_for(int64_t n, int threads = 256, int cap = 65536) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-void trsm_f64_cm(int side, int uplo, int transa, int diag, int64_t m, int64_t n, double alpha,
-                 const double* a, int64_t lda, double* b, int64_t ldb, hipStream_t s) {
+template <typename T>
+void trsm_cm(int side, int uplo, int transa, int diag, int64_t m, int64_t n, T alpha, const T* a,
+             int64_t lda, T* b, int64_t ldb, hipStream_t s) {
   if (m <= 0 || n <= 0) return;
-  if (alpha == 0.0) { scale_f64_cm(m, n, 0.0, b, ldb, s); return; }   // trsm.zig:99-113
-  if (alpha != 1.0) scale_f64_cm(m, n, alpha, b, ldb, s);
+  if (alpha == T(0)) { scale_cm<T>(m, n, T(0), b, ldb, s, 0); return; }   // trsm.zig:99-113
+  if (alpha != T(1)) scale_cm<T>(m, n, alpha, b, ldb, s, 0);
   const bool nt = is_notrans(transa);
   const bool unit = diag == kUnit;
   if (side == kLeft) {
@@ -236,15 +244,15 @@ void trsm_f64_cm(int side, int uplo, int transa, int diag, int64_t m, int64_t n,
     for (int64_t bi = 0; bi < nblk; ++bi) {
       const int64_t blk = fwd ? bi : nblk - 1 - bi;
       const int64_t i0 = blk * TB, nb = std::min<int64_t>(TB, m - i0);
-      hipLaunchKernelGGL((trsm_diag_kernel<true>), dim3(grid_for(n)), dim3(256), 0, s, (int)nb,
+      hipLaunchKernelGGL((trsm_diag_kernel<T, true>), dim3(grid_for(n)), dim3(256), 0, s, (int)nb,
                          a + i0 + i0 * lda, lda, nt ? 0 : 1, fwd ? 1 : 0, unit ? 1 : 0,
                          b + i0, ldb, n);
       ZH_LAUNCH_CHECK();
       // update the remaining rows: B_rest -= op(A)_{rest, blk} X_blk
       const int64_t r0 = fwd ? i0 + nb : 0, rn = fwd ? m - (i0 + nb) : i0;
       if (rn > 0) {
-        if (nt) gemm_f64_cm(kNoTrans, kNoTrans, rn, n, nb, -1.0, a + r0 + i0 * lda, lda, b + i0, ldb, 1.0, b + r0, ldb, s);
-        else gemm_f64_cm(kTrans, kNoTrans, rn, n, nb, -1.0, a + i0 + r0 * lda, lda, b + i0, ldb, 1.0, b + r0, ldb, s);
+        if (nt) gemm_cm<T>(kNoTrans, kNoTrans, rn, n, nb, T(-1), a + r0 + i0 * lda, lda, b + i0, ldb, T(1), b + r0, ldb, s, 0);
+        else gemm_cm<T>(kTrans, kNoTrans, rn, n, nb, T(-1), a + i0 + r0 * lda, lda, b + i0, ldb, T(1), b + r0, ldb, s, 0);
       }
     }
   } else {
@@ -255,25 +263,44 @@ void trsm_f64_cm(int side, int uplo, int transa, int diag, int64_t m, int64_t n,
       const int64_t blk = fwd ? bj : nblk - 1 - bj;
       const int64_t j0 = blk * TB, nb = std::min<int64_t>(TB, n - j0);
       // rows of B solve x M = b, i.e. M^T x^T = b^T with M = op(A_jj)
-      hipLaunchKernelGGL((trsm_diag_kernel<false>), dim3(grid_for(m)), dim3(256), 0, s, (int)nb,
+      hipLaunchKernelGGL((trsm_diag_kernel<T, false>), dim3(grid_for(m)), dim3(256), 0, s, (int)nb,
                          a + j0 + j0 * lda, lda, nt ? 1 : 0, fwd ? 1 : 0, unit ? 1 : 0,
                          b + j0 * ldb, ldb, m);
       ZH_LAUNCH_CHECK();
       const int64_t c0 = fwd ? j0 + nb : 0, cn = fwd ? n - (j0 + nb) : j0;
       if (cn > 0) {
-        if (nt) gemm_f64_cm(kNoTrans, kNoTrans, m, cn, nb, -1.0, b + j0 * ldb, ldb, a + j0 + c0 * lda, lda, 1.0, b + c0 * ldb, ldb, s);
-        else gemm_f64_cm(kNoTrans, kTrans, m, cn, nb, -1.0, b + j0 * ldb, ldb, a + c0 + j0 * lda, lda, 1.0, b + c0 * ldb, ldb, s);
+        if (nt) gemm_cm<T>(kNoTrans, kNoTrans, m, cn, nb, T(-1), b + j0 * ldb, ldb, a + j0 + c0 * lda, lda, T(1), b + c0 * ldb, ldb, s, 0);
+        else gemm_cm<T>(kNoTrans, kTrans, m, cn, nb, T(-1), b + j0 * ldb, ldb, a + c0 + j0 * lda, lda, T(1), b + c0 * ldb, ldb, s, 0);
       }
     }
   }
 }
+template void trsm_cm<double>(int, int, int, int, int64_t, int64_t, double, const double*, int64_t, double*,
+                              int64_t, hipStream_t);
+template void trsm_cm<float>(int, int, int, int, int64_t, int64_t, float, const float*, int64_t, float*,
+                             int64_t, hipStream_t);
 
-void syrk_f64_cm(int uplo, int trans, int64_t n, int64_t k, double alpha, const double* a,
-                 int64_t lda, double beta, double* c, int64_t ldc, hipStream_t s) {
+void trsm_f64_cm(int side, int uplo, int transa, int diag, int64_t m, int64_t n, double alpha,
+                 const double* a, int64_t lda, double* b, int64_t ldb, hipStream_t s) {
+  trsm_cm<double>(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, s);
+}
+
+template <typename T>
+void syrk_cm(int uplo, int trans, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, T beta, T* c,
+             int64_t ldc, hipStream_t s) {
   if (n <= 0) return;
   const int ta = is_notrans(trans) ? kNoTrans : kTrans;
   const int tb = is_notrans(trans) ? kTrans : kNoTrans;
-  gemm_f64_cm(ta, tb, n, n, k, alpha, a, lda, a, lda, beta, c, ldc, s, uplo);
+  gemm_cm<T>(ta, tb, n, n, k, alpha, a, lda, a, lda, beta, c, ldc, s, uplo);
+}
+template void syrk_cm<double>(int, int, int64_t, int64_t, double, const double*, int64_t, double, double*,
+                              int64_t, hipStream_t);
+template void syrk_cm<float>(int, int, int64_t, int64_t, float, const float*, int64_t, float, float*,
+                             int64_t, hipStream_t);
+
+void syrk_f64_cm(int uplo, int trans, int64_t n, int64_t k, double alpha, const double* a,
+                 int64_t lda, double beta, double* c, int64_t ldc, hipStream_t s) {
+  syrk_cm<double>(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, s);
 }
 
 void laswp_rm(int64_t ncols, double* a, int64_t lda, int k1, int k2, const int32_t* ipiv, int inc,
@@ -294,21 +321,26 @@ void laswp_cm(int64_t ncols, double* a, int64_t lda, int k1, int k2, const int32
   ZH_LAUNCH_CHECK();
 }
 
-void transpose_square_inplace(int64_t n, double* a, int64_t ld, hipStream_t s) {
+template <typename T>
+void transpose_square_inplace(int64_t n, T* a, int64_t ld, hipStream_t s) {
   if (n <= 1) return;
   const unsigned t = (unsigned)((n + TT - 1) / TT);
-  hipLaunchKernelGGL(transpose_square_kernel, dim3(t, t), dim3(TT, 8), 0, s, n, a, ld);
+  hipLaunchKernelGGL(transpose_square_kernel<T>, dim3(t, t), dim3(TT, 8), 0, s, n, a, ld);
   ZH_LAUNCH_CHECK();
 }
+template void transpose_square_inplace<double>(int64_t, double*, int64_t, hipStream_t);
+template void transpose_square_inplace<float>(int64_t, float*, int64_t, hipStream_t);
 
-void transpose_copy(int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst,
-                    int64_t ldd, hipStream_t s) {
+template <typename T>
+void transpose_copy(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s) {
   if (rows <= 0 || cols <= 0) return;
-  hipLaunchKernelGGL(transpose_copy_kernel,
+  hipLaunchKernelGGL(transpose_copy_kernel<T>,
                      dim3((unsigned)((cols + TT - 1) / TT), (unsigned)((rows + TT - 1) / TT)),
                      dim3(TT, 8), 0, s, rows, cols, src, lds, dst, ldd);
   ZH_LAUNCH_CHECK();
 }
+template void transpose_copy<double>(int64_t, int64_t, const double*, int64_t, double*, int64_t, hipStream_t);
+template void transpose_copy<float>(int64_t, int64_t, const float*, int64_t, float*, int64_t, hipStream_t);
 
 // ---- Level 1 -----------------------------------------------------------------
 void iamax_f64(int64_t n, const double* x, int64_t incx, int32_t* out, hipStream_t s) {
@@ -329,6 +361,165 @@ void scal_f64(int64_t n, double alpha, double* x, int64_t incx, hipStream_t s) {
 void swap_f64(int64_t n, double* x, int64_t incx, double* y, int64_t incy, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(swap_kernel, dim3(grid_for(n)), dim3(256), 0, s, n, x, incx, y, incy);
+  ZH_LAUNCH_CHECK();
+}
+
+}  // namespace zmlhip
+
+// ---- TRSV and LANGE (column-major view) ----------------------------------------
+namespace zmlhip {
+namespace {
+
+__global__ void gather_kernel(int64_t n, const double* __restrict__ x, int64_t incx, double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t kx = incx < 0 ? (-n + 1) * incx : 0;
+  y[i] = x[kx + i * incx];
+}
+__global__ void scatter_kernel(int64_t n, const double* __restrict__ y, double* __restrict__ x, int64_t incx) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t kx = incx < 0 ? (-n + 1) * incx : 0;
+  x[kx + i * incx] = y[i];
+}
+
+// max that keeps NaN (LAPACK's dlange propagates a NaN entry)
+__device__ __forceinline__ double nmax(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// Block reduction of v (max or sum) into thread 0's return value (256 threads).
+__device__ __forceinline__ double block_reduce(double v, bool sum) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off);
+    v = sum ? v + o : nmax(v, o);
+  }
+  __shared__ double sw[4];
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = sw[0];
+  for (int w = 1; w < 4; ++w) t = sum ? t + sw[w] : nmax(t, sw[w]);
+  return t;
+}
+
+// part[block] = max |a_ij| (sq == 0) or sum (|a_ij| / *scale)^2 (sq == 1) over
+// the elements the block visits (i fastest, so the reads are coalesced).
+__global__ void lange_elem_kernel(int sq, int64_t m, int64_t n, const double* __restrict__ a, int64_t lda,
+                                  const double* __restrict__ scale, double* __restrict__ part) {
+  const double inv = sq ? 1.0 / *scale : 1.0;
+  double acc = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % m, j = e / m;
+    const double v = fabs(a[i + j * lda]);
+    if (sq) { const double q = v * inv; acc += q * q; }
+    else acc = nmax(acc, v);
+  }
+  const double t = block_reduce(acc, sq != 0);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// colsum[j] = sum_i |a_ij| (one block per column, ascending partial sums per thread)
+__global__ void lange_colsum_kernel(int64_t m, const double* __restrict__ a, int64_t lda, double* __restrict__ colsum) {
+  const double* col = a + blockIdx.x * lda;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < m; i += blockDim.x) acc += fabs(col[i]);
+  const double t = block_reduce(acc, true);
+  if (threadIdx.x == 0) colsum[blockIdx.x] = t;
+}
+
+// part[s * m + i] = sum over the columns of split s of |a_ij|
+__global__ void lange_rowsum_partial(int64_t m, int64_t n, const double* __restrict__ a, int64_t lda,
+                                     double* __restrict__ part, int64_t cps) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t sidx = blockIdx.y, j0 = sidx * cps, j1 = j0 + cps < n ? j0 + cps : n;
+  if (i >= m) return;
+  double acc = 0.0;
+  for (int64_t j = j0; j < j1; ++j) acc += fabs(a[i + j * lda]);
+  part[sidx * m + i] = acc;
+}
+__global__ void lange_rowsum_final(int64_t m, int nsplit, const double* __restrict__ part, double* __restrict__ rows) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double acc = 0.0;
+  for (int sp = 0; sp < nsplit; ++sp) acc += part[sp * m + i];
+  rows[i] = acc;
+}
+
+// *out = max (or sum) of x[0..cnt); mode 2: *out = *scale * sqrt(sum) (the
+// Frobenius norm from its scaled sum of squares; a zero, infinite or NaN scale
+// is the result itself)
+__global__ void lange_final_kernel(int mode, int64_t cnt, const double* __restrict__ x,
+                                   const double* __restrict__ scale, double* __restrict__ out) {
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < cnt; e += blockDim.x) acc = mode == 0 ? nmax(acc, x[e]) : acc + x[e];
+  const double t = block_reduce(acc, mode != 0);
+  if (threadIdx.x == 0) {
+    if (mode == 2) {
+      const double sc = *scale;
+      *out = (sc == 0.0 || !(sc < INFINITY)) ? sc : sc * sqrt(t);
+    } else {
+      *out = t;
+    }
+  }
+}
+
+}  // namespace
+
+void trsv_f64_cm(int uplo, int trans, int diag, int64_t n, const double* a, int64_t lda, double* x,
+                 int64_t incx, hipStream_t s) {
+  if (n <= 0) return;
+  // the vector as an n x 1 matrix: the left-side TRSM divides by the diagonal,
+  // as trsv.zig does in every branch
+  double* y = static_cast<double*>(workspace(sizeof(double) * (size_t)n, 19));
+  hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, (const double*)x, incx, y);
+  ZH_LAUNCH_CHECK();
+  trsm_cm<double>(kLeft, uplo, trans, diag, n, 1, 1.0, a, lda, y, n, s);
+  hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, (const double*)y, x, incx);
+  ZH_LAUNCH_CHECK();
+}
+
+void lange_f64_cm(char norm, int64_t m, int64_t n, const double* a, int64_t lda, double* out, hipStream_t s) {
+  constexpr int NPART = 256;
+  const int64_t nsplit_max = 64;
+  double* w = static_cast<double*>(workspace(sizeof(double) * (NPART + 8 + (size_t)std::max<int64_t>(m, n) *
+                                                                 (nsplit_max + 1)), 20));
+  double* part = w;             // NPART block results
+  double* scal = w + NPART;     // Frobenius scale
+  double* big = w + NPART + 8;  // column sums, or row-sum partials + row sums
+  const unsigned eb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(NPART, (m * n + 255) / 256));
+  switch (norm) {
+    case 'M': case 'm':
+      hipLaunchKernelGGL(lange_elem_kernel, dim3(eb), dim3(256), 0, s, 0, m, n, a, lda, (const double*)nullptr, part);
+      hipLaunchKernelGGL(lange_final_kernel, dim3(1), dim3(256), 0, s, 0, (int64_t)eb, (const double*)part,
+                         (const double*)nullptr, out);
+      break;
+    case 'F': case 'f': case 'E': case 'e':
+      hipLaunchKernelGGL(lange_elem_kernel, dim3(eb), dim3(256), 0, s, 0, m, n, a, lda, (const double*)nullptr, part);
+      hipLaunchKernelGGL(lange_final_kernel, dim3(1), dim3(256), 0, s, 0, (int64_t)eb, (const double*)part,
+                         (const double*)nullptr, scal);
+      hipLaunchKernelGGL(lange_elem_kernel, dim3(eb), dim3(256), 0, s, 1, m, n, a, lda, (const double*)scal, part);
+      hipLaunchKernelGGL(lange_final_kernel, dim3(1), dim3(256), 0, s, 2, (int64_t)eb, (const double*)part,
+                         (const double*)scal, out);
+      break;
+    case 'O': case 'o': case '1':
+      hipLaunchKernelGGL(lange_colsum_kernel, dim3((unsigned)n), dim3(256), 0, s, m, a, lda, big);
+      hipLaunchKernelGGL(lange_final_kernel, dim3(1), dim3(256), 0, s, 0, n, (const double*)big,
+                         (const double*)nullptr, out);
+      break;
+    case 'I': case 'i': {
+      const int64_t rb = (m + 255) / 256;
+      const int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>({nsplit_max, n, (2048 + rb - 1) / rb}));
+      const int64_t cps = (n + nsplit - 1) / nsplit;
+      double* rows = big + nsplit * m;
+      hipLaunchKernelGGL(lange_rowsum_partial, dim3((unsigned)rb, (unsigned)nsplit), dim3(256), 0, s, m, n, a, lda,
+                         big, cps);
+      hipLaunchKernelGGL(lange_rowsum_final, dim3((unsigned)rb), dim3(256), 0, s, m, (int)nsplit, (const double*)big,
+                         rows);
+      hipLaunchKernelGGL(lange_final_kernel, dim3(1), dim3(256), 0, s, 0, m, (const double*)rows,
+                         (const double*)nullptr, out);
+      break;
+    }
+    default:
+      throw std::runtime_error("lange: unknown norm");
+  }
   ZH_LAUNCH_CHECK();
 }
 

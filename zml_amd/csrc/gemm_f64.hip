@@ -1,4 +1,5 @@
-// FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4_f64), column-major view.
+// FP64 / FP32 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4_f64 and
+// v_mfma_f32_16x16x4_f32), column-major view.
 //
 //   C = alpha * op(A) * op(B) + beta * C        (reference blas/gemm.zig:68-1033)
 //
@@ -14,7 +15,9 @@
 // so every MFMA operand fetch is a 16-lane contiguous read). The MFMA is
 // issued as C^T = op(B)^T op(A)^T so that the 16 lanes of an output group
 // hold 16 consecutive rows of one column of C: stores and beta-loads are
-// 128-byte contiguous in column-major memory.
+// 128-byte contiguous in column-major memory. The two element types differ
+// only in the MFMA and in which output row a lane's accumulator r holds
+// (FP64: lane / 16 + 4 r; FP32: 4 (lane / 16) + r).
 #include "zmlhip_internal.hpp"
 #include "tiles.hpp"
 
@@ -22,6 +25,25 @@ namespace zmlhip {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// The MFMA of each element type and its accumulator layout.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4 acc;
+  static __device__ __forceinline__ acc step(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // column (within the 16 x 16 output tile) of accumulator r of a lane group lk
+  static __device__ __forceinline__ int col(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct Mfma<float> {
+  typedef f4 acc;
+  static __device__ __forceinline__ acc step(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int col(int lk, int r) { return 4 * lk + r; }
+};
 
 constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
 constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
@@ -48,13 +70,14 @@ __device__ __forceinline__ void tile_coords(int64_t bid, int64_t ntm, int64_t nt
   tn = r / gm;
 }
 
-template <bool TA, bool TB>
+template <typename T, bool TA, bool TB>
 __global__ void __launch_bounds__(NT, 2)
-gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __restrict__ A,
-                int64_t lda, const double* __restrict__ B, int64_t ldb, double beta,
-                double* __restrict__ C, int64_t ldc, int tri, int64_t tri_tiles) {
-  __shared__ double As[2][BK * LDA_S];
-  __shared__ double Bs[2][BK * LDB_S];
+gemm_kernel(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A,
+            int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
+            T* __restrict__ C, int64_t ldc, int tri, int64_t tri_tiles) {
+  typedef typename Mfma<T>::acc acc_t;
+  __shared__ T As[2][BK * LDA_S];
+  __shared__ T Bs[2][BK * LDB_S];
 
   const int64_t ntm = (m + BM - 1) / BM, ntn = (n + BN - 1) / BN;
   int64_t tm, tn;
@@ -69,7 +92,7 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
   const int wm = wave & 1, wn = wave >> 1;
 
   // ---- global -> register staging (8 elements of A and 8 of B per thread)
-  double ra[8], rb[8];
+  T ra[8], rb[8];
   auto load_tiles = [&](int64_t k0) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -77,7 +100,7 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
       if (!TA) { i = row0 + (t & 127); l = k0 + (t >> 7) + 2 * q; }   // m-contiguous
       else     { l = k0 + (t & 15);    i = row0 + (t >> 4) + 16 * q; } // k-contiguous
       const bool ok = i < m && l < k;
-      ra[q] = ok ? (TA ? A[l + i * lda] : A[i + l * lda]) : 0.0;
+      ra[q] = ok ? (TA ? A[l + i * lda] : A[i + l * lda]) : T(0);
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -85,14 +108,14 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
       if (TB) { j = col0 + (t & 127); l = k0 + (t >> 7) + 2 * q; }    // n-contiguous
       else    { l = k0 + (t & 15);    j = col0 + (t >> 4) + 16 * q; }  // k-contiguous
       const bool ok = j < n && l < k;
-      rb[q] = ok ? (TB ? B[j + l * ldb] : B[l + j * ldb]) : 0.0;
+      rb[q] = ok ? (TB ? B[j + l * ldb] : B[l + j * ldb]) : T(0);
     }
   };
   // Update form C <- C +- op(A) op(B) (beta 1, alpha +-1): the accumulators start
   // from C, loaded with the first operand slab, and B is staged negated for
   // alpha -1, so the epilogue only stores.
-  const bool cinit = beta == 1.0 && (alpha == 1.0 || alpha == -1.0);
-  const bool negb = cinit && alpha == -1.0;
+  const bool cinit = beta == T(1) && (alpha == T(1) || alpha == T(-1));
+  const bool negb = cinit && alpha == T(-1);
   auto store_tiles = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -111,15 +134,15 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
   };
 
   const int lr = lane & 15, lk = lane >> 4;
-  // lane holds C[i0 + lr][j0 + lk + 4r] of each 16 x 16 tile
+  // lane holds C[i0 + lr][j0 + Mfma<T>::col(lk, r)] of each 16 x 16 tile
   auto c_ok = [&](int64_t i, int64_t j) {
     return i < m && j < n && !(tri == kLower && i < j) && !(tri == kUpper && i > j);
   };
-  d4 acc[4][4];
+  acc_t acc[4][4];
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
-    for (int y = 0; y < 4; ++y) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int y = 0; y < 4; ++y) acc[x][y] = acc_t{T(0), T(0), T(0), T(0)};
   if (cinit) {
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
@@ -128,7 +151,7 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
       for (int y = 0; y < 4; ++y)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t j = col0 + wn * 64 + 16 * y + lk + 4 * r;
+          const int64_t j = col0 + wn * 64 + 16 * y + Mfma<T>::col(lk, r);
           if (c_ok(i, j)) acc[x][y][r] = C[i + j * ldc];
         }
     }
@@ -142,12 +165,12 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) load_tiles((int64_t)(kt + 1) * BK);
-    const double* as = As[cur] + wm * 64 + lr;
-    const double* bs = Bs[cur] + wn * 64 + lr;
+    const T* as = As[cur] + wm * 64 + lr;
+    const T* bs = Bs[cur] + wn * 64 + lr;
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kr = 4 * ks + lk;
-      double av[4], bv[4];
+      T av[4], bv[4];
 #pragma unroll
       for (int x = 0; x < 4; ++x) av[x] = as[kr * LDA_S + 16 * x];
 #pragma unroll
@@ -156,7 +179,7 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
       for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y)
-          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[y], av[x], acc[x][y], 0, 0, 0);
+          acc[x][y] = Mfma<T>::step(bv[y], av[x], acc[x][y]);
     }
     if (kt + 1 < nk) store_tiles(cur ^ 1);
     __syncthreads();
@@ -171,7 +194,7 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
       for (int y = 0; y < 4; ++y)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t j = col0 + wn * 64 + 16 * y + lk + 4 * r;
+          const int64_t j = col0 + wn * 64 + 16 * y + Mfma<T>::col(lk, r);
           if (c_ok(i, j)) C[i + j * ldc] = acc[x][y][r];
         }
     }
@@ -179,19 +202,19 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
   }
   // general alpha, beta: per row tile the 16 C values are all requested before any
   // is used (one memory latency per row tile instead of one per element)
-  const bool need_c = beta != 0.0;
+  const bool need_c = beta != T(0);
 #pragma unroll
   for (int x = 0; x < 4; ++x) {
     const int64_t i = row0 + wm * 64 + 16 * x + lr;
     bool ok[4][4];
-    double cv[4][4];
+    T cv[4][4];
 #pragma unroll
     for (int y = 0; y < 4; ++y) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t j = col0 + wn * 64 + 16 * y + lk + 4 * r;
+        const int64_t j = col0 + wn * 64 + 16 * y + Mfma<T>::col(lk, r);
         ok[y][r] = i < m && j < n && !(tri == kLower && i < j) && !(tri == kUpper && i > j);
-        cv[y][r] = (need_c && ok[y][r]) ? C[i + j * ldc] : 0.0;
+        cv[y][r] = (need_c && ok[y][r]) ? C[i + j * ldc] : T(0);
       }
     }
 #pragma unroll
@@ -199,44 +222,46 @@ gemm_f64_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __r
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (!ok[y][r]) continue;
-        const int64_t j = col0 + wn * 64 + 16 * y + lk + 4 * r;
-        const double v = alpha * acc[x][y][r];
-        C[i + j * ldc] = beta == 0.0 ? v : (beta == 1.0 ? cv[y][r] + v : beta * cv[y][r] + v);
+        const int64_t j = col0 + wn * 64 + 16 * y + Mfma<T>::col(lk, r);
+        const T v = alpha * acc[x][y][r];
+        C[i + j * ldc] = beta == T(0) ? v : (beta == T(1) ? cv[y][r] + v : beta * cv[y][r] + v);
       }
     }
   }
 }
 
-__global__ void scale_kernel(int64_t m, int64_t n, double beta, double* __restrict__ c,
-                             int64_t ldc, int tri) {
+template <typename T>
+__global__ void scale_kernel(int64_t m, int64_t n, T beta, T* __restrict__ c, int64_t ldc, int tri) {
   const int64_t total = m * n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = e % m, j = e / m;
     if (tri == kLower && i < j) continue;
     if (tri == kUpper && i > j) continue;
-    double* p = c + i + j * ldc;
-    *p = beta == 0.0 ? 0.0 : *p * beta;
+    T* p = c + i + j * ldc;
+    *p = beta == T(0) ? T(0) : *p * beta;
   }
 }
 
 }  // namespace
 
-void scale_f64_cm(int64_t m, int64_t n, double beta, double* c, int64_t ldc, hipStream_t s,
-                  int tri) {
-  if (m <= 0 || n <= 0 || beta == 1.0) return;
+template <typename T>
+void scale_cm(int64_t m, int64_t n, T beta, T* c, int64_t ldc, hipStream_t s, int tri) {
+  if (m <= 0 || n <= 0 || beta == T(1)) return;
   int64_t total = m * n;
   int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
-  hipLaunchKernelGGL(scale_kernel, dim3(blocks), dim3(256), 0, s, m, n, beta, c, ldc, tri);
+  hipLaunchKernelGGL(scale_kernel<T>, dim3(blocks), dim3(256), 0, s, m, n, beta, c, ldc, tri);
   ZH_LAUNCH_CHECK();
 }
+template void scale_cm<double>(int64_t, int64_t, double, double*, int64_t, hipStream_t, int);
+template void scale_cm<float>(int64_t, int64_t, float, float*, int64_t, hipStream_t, int);
 
-void gemm_f64_cm(int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
-                 const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
-                 double* c, int64_t ldc, hipStream_t s, int tri) {
+template <typename T>
+void gemm_cm(int transa, int transb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda,
+             const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s, int tri) {
   if (m <= 0 || n <= 0) return;
-  if (alpha == 0.0 || k <= 0) {  // gemm.zig:104-140: only the beta scaling remains
-    scale_f64_cm(m, n, beta, c, ldc, s, tri);
+  if (alpha == T(0) || k <= 0) {  // gemm.zig:104-140: only the beta scaling remains
+    scale_cm<T>(m, n, beta, c, ldc, s, tri);
     return;
   }
   const bool ta = !is_notrans(transa), tb = !is_notrans(transb);
@@ -244,11 +269,25 @@ void gemm_f64_cm(int transa, int transb, int64_t m, int64_t n, int64_t k, double
   const int64_t tri_tiles = tri_tile_count(tri, ntm, ntn);
   const int64_t tiles = tri_tiles > 0 ? tri_tiles : ntm * ntn;
   dim3 grid((unsigned)tiles), block(NT);
-  if (!ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
-  else if (ta && !tb) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
-  else if (!ta && tb) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
-  else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
+  if (!ta && !tb) hipLaunchKernelGGL((gemm_kernel<T, false, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
+  else if (ta && !tb) hipLaunchKernelGGL((gemm_kernel<T, true, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
+  else if (!ta && tb) hipLaunchKernelGGL((gemm_kernel<T, false, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
+  else hipLaunchKernelGGL((gemm_kernel<T, true, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tri, tri_tiles);
   ZH_LAUNCH_CHECK();
+}
+template void gemm_cm<double>(int, int, int64_t, int64_t, int64_t, double, const double*, int64_t, const double*,
+                              int64_t, double, double*, int64_t, hipStream_t, int);
+template void gemm_cm<float>(int, int, int64_t, int64_t, int64_t, float, const float*, int64_t, const float*,
+                             int64_t, float, float*, int64_t, hipStream_t, int);
+
+void scale_f64_cm(int64_t m, int64_t n, double beta, double* c, int64_t ldc, hipStream_t s, int tri) {
+  scale_cm<double>(m, n, beta, c, ldc, s, tri);
+}
+
+void gemm_f64_cm(int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                 double* c, int64_t ldc, hipStream_t s, int tri) {
+  gemm_cm<double>(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri);
 }
 
 }  // namespace zmlhip

@@ -1,4 +1,4 @@
-// FP64 LU with partial pivoting (zml lapack/getrf.zig:14-152, getrf2.zig:14-210).
+// LU with partial pivoting, FP64 and FP32 (zml lapack/getrf.zig:14-152, getrf2.zig:14-210).
 //
 // Storage: the factorization works on a row-major matrix (rows contiguous) so
 // that row interchanges move contiguous memory. Column-major callers are
@@ -31,19 +31,9 @@ namespace zmlhip {
 
 constexpr int NB_F64 = 512;     // block column width of the FP64 driver
 
-// Row-major wrappers over the column-major kernels (zml's own mappings:
-// gemm.zig:48-65, trsm.zig:48-62).
-static void gemm_rm_nn(int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
-                       const double* b, int64_t ldb, double beta, double* c, int64_t ldc,
-                       hipStream_t s) {
-  gemm_f64_cm(kNoTrans, kNoTrans, n, m, k, alpha, b, ldb, a, lda, beta, c, ldc, s);
-}
-static void trsm_rm_llnu(int64_t m, int64_t n, const double* a, int64_t lda, double* b, int64_t ldb,
-                         hipStream_t s) {
-  trsm_f64_cm(kRight, kUpper, kNoTrans, kUnit, n, m, 1.0, a, lda, b, ldb, s);
-}
-
-// Trailing-update kernels of a row-major LU: C -= L U and B <- L11^-1 B.
+// Trailing-update kernels of a row-major LU: C -= L U and B <- L11^-1 B, each a
+// column-major kernel on the transposed view (zml's own mappings: gemm.zig:48-65,
+// trsm.zig:48-62).
 // The wide U12 solve goes through the inverse of the unit lower L11: the
 // panel stream forms L11^-1 right after factoring the block column, and the
 // main stream then computes U12 = L11^-1 A12 as one GEMM on a copy of A12
@@ -96,7 +86,7 @@ void leaf_factor(int64_t m, T* A, int64_t lda, int64_t r0, int64_t c0, int w, in
     const uint32_t base = (uint32_t)st.launches * (uint32_t)(panel::W + 1);
     st.launches++;
     hipLaunchKernelGGL((panel::leaf_kernel<T, true>), dim3(G), dim3(PT), 0, s, M, w,
-                       A + r0 * lda + c0, lda, ipiv + r0, r0, info, c0, w_, base, 2000000L);
+                       A + r0 * lda + c0, lda, ipiv + r0, r0, info, c0, w_, base, panel_spin_limit());
   }
   ZH_LAUNCH_CHECK();
 }
@@ -153,12 +143,14 @@ void factor_panel(int64_t m, T* A, int64_t lda, int64_t j0, int64_t jb, int32_t*
 // Row-major right-looking LU with look-ahead of depth one: while the main
 // stream applies step b to the columns beyond block b+1, the panel stream
 // already factors block b+1 (whose columns step b updated first). ipiv
-// (1-based) and info live in device memory; info must be zero on entry.
+// (1-based) and info live in device memory; info must be zero on entry. When a
+// panel's bounded wait gave up, info ends as kErrSync (the factors are invalid).
 template <typename T>
 void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info, int64_t nb,
                      const LuOps<T>& ops, hipStream_t s) {
   const int64_t mn = std::min(m, n);
   if (mn <= 0) return;
+  clear_leaf_error(s);
   hipStream_t sp = aux_stream(0);
   hipEvent_t ev_in = aux_event(0), ev_panel = aux_event(1), ev_next = aux_event(2);
   const hipStream_t caller = s;
@@ -235,64 +227,88 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
     ZH_CHECK(hipEventRecord(ev_out, s));
     ZH_CHECK(hipStreamWaitEvent(caller, ev_out, 0));
   }
+  report_leaf_error(info, caller);
 }
 
-static void gemm_f64_update(int64_t rows, int64_t cols, int64_t kb, const double* L, int64_t ldl,
-                            const double* U, int64_t ldu, double* C, int64_t ldc, hipStream_t s) {
-  void* tok = profile_begin(kProfGemmF64, s);
-  gemm_rm_nn(rows, cols, kb, -1.0, L, ldl, U, ldu, 1.0, C, ldc, s);
+template <typename T>
+static void gemm_update(int64_t rows, int64_t cols, int64_t kb, const T* L, int64_t ldl, const T* U,
+                        int64_t ldu, T* C, int64_t ldc, hipStream_t s) {
+  void* tok = sizeof(T) == 8 ? profile_begin(kProfGemmF64, s) : nullptr;
+  // row-major C -= L U is column-major C^T -= U^T L^T (gemm.zig:48-65)
+  gemm_cm<T>(kNoTrans, kNoTrans, cols, rows, kb, T(-1), U, ldu, L, ldl, T(1), C, ldc, s, 0);
   // algorithmic traffic: L and U read once, C read and written once
-  profile_end(tok, s, 2.0 * rows * cols * kb, 8.0 * (rows * kb + kb * cols + 2.0 * rows * cols));
+  profile_end(tok, s, 2.0 * rows * cols * kb, (double)sizeof(T) * (rows * kb + kb * cols + 2.0 * rows * cols));
 }
-static void trsm_f64_update(int64_t kb, int64_t cols, const double* L11, int64_t ld, double* B,
-                            int64_t ldb, hipStream_t s) {
-  trsm_rm_llnu(kb, cols, L11, ld, B, ldb, s);
+template <typename T>
+static void trsm_update(int64_t kb, int64_t cols, const T* L11, int64_t ld, T* B, int64_t ldb, hipStream_t s) {
+  trsm_cm<T>(kRight, kUpper, kNoTrans, kUnit, cols, kb, T(1), L11, ld, B, ldb, s);
 }
 
-__global__ void set_identity_kernel(int64_t n, double* __restrict__ a) {
+template <typename T>
+__global__ void set_identity_kernel(int64_t n, T* __restrict__ a) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e < n * n) a[e] = (e / n == e % n) ? 1.0 : 0.0;
+  if (e < n * n) a[e] = (e / n == e % n) ? T(1) : T(0);
 }
-static void invert_f64(int64_t kb, const double* L11, int64_t ld, double* inv, hipStream_t s) {
-  hipLaunchKernelGGL(set_identity_kernel, dim3((unsigned)((kb * kb + 255) / 256)), dim3(256), 0, s, kb, inv);
+template <typename T>
+static void invert_unit_lower(int64_t kb, const T* L11, int64_t ld, T* inv, hipStream_t s) {
+  hipLaunchKernelGGL(set_identity_kernel<T>, dim3((unsigned)((kb * kb + 255) / 256)), dim3(256), 0, s, kb, inv);
   ZH_LAUNCH_CHECK();
-  trsm_rm_llnu(kb, kb, L11, ld, inv, kb, s);
+  trsm_update<T>(kb, kb, L11, ld, inv, kb, s);
 }
-static void apply_inv_f64(int64_t kb, int64_t cols, const double* inv, double* B, int64_t ldb,
-                          double* tmp, hipStream_t s) {
-  ZH_CHECK(hipMemcpy2DAsync(tmp, sizeof(double) * cols, B, sizeof(double) * ldb, sizeof(double) * cols,
-                            kb, hipMemcpyDeviceToDevice, s));
-  gemm_rm_nn(kb, cols, kb, 1.0, inv, kb, tmp, cols, 0.0, B, ldb, s);
+template <typename T>
+static void apply_inv(int64_t kb, int64_t cols, const T* inv, T* B, int64_t ldb, T* tmp, hipStream_t s) {
+  ZH_CHECK(hipMemcpy2DAsync(tmp, sizeof(T) * cols, B, sizeof(T) * ldb, sizeof(T) * cols, kb,
+                            hipMemcpyDeviceToDevice, s));
+  gemm_cm<T>(kNoTrans, kNoTrans, cols, kb, kb, T(1), tmp, cols, inv, kb, T(0), B, ldb, s, 0);
 }
 
-void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, int* info,
-                  hipStream_t s) {
-  // The pivot leaf keeps one 256-row workgroup per CU resident for the whole
-  // panel; when its first grid fits in a quarter of the CUs, those CUs are kept
-  // free of trailing updates so the leaf never waits for GEMM tiles to drain.
+// Plain partial-pivoting LU of a row-major matrix in T (FP64 for dgetrf, FP32
+// for sgetrf): the same driver, leaves and kernels, no low-precision update.
+template <typename T>
+void getrf_rm(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info, hipStream_t s) {
+  // The pivot leaf keeps one workgroup per CU resident for the whole panel;
+  // when its first grid fits in a quarter of the CUs, those CUs are kept free
+  // of trailing updates so the leaf never waits for GEMM tiles to drain.
   // (rounded up to a power of two from 8, so few distinct masked streams exist)
-  const int leaf_wgs = (int)((m + panel::pt_of<double>() - 1) / panel::pt_of<double>());
+  const int leaf_wgs = (int)((m + panel::pt_of<T>() - 1) / panel::pt_of<T>());
   int spare = 0;
   if (leaf_wgs > 1) {
     spare = 8;
     while (spare < leaf_wgs) spare *= 2;
     if (spare > device_cu_count() / 4) spare = 0;
   }
-  const LuOps<double> ops{gemm_f64_update, trsm_f64_update, invert_f64, apply_inv_f64, nullptr, nullptr,
-                          nullptr, spare, nullptr};
-  getrf_lookahead<double>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
+  const LuOps<T> ops{gemm_update<T>, trsm_update<T>, invert_unit_lower<T>, apply_inv<T>, nullptr, nullptr,
+                     nullptr, spare, nullptr};
+  getrf_lookahead<T>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
+}
+template void getrf_rm<float>(int64_t, int64_t, float*, int64_t, int32_t*, int*, hipStream_t);
+template void getrf_rm<double>(int64_t, int64_t, double*, int64_t, int32_t*, int*, hipStream_t);
+
+void getrf_f64_rm(int64_t m, int64_t n, double* A, int64_t lda, int32_t* ipiv, int* info,
+                  hipStream_t s) {
+  getrf_rm<double>(m, n, A, lda, ipiv, info, s);
 }
 
 template void getrf_lookahead<float>(int64_t, int64_t, float*, int64_t, int32_t*, int*, int64_t,
                                      const LuOps<float>&, hipStream_t);
 
-int leaf_error_flag(hipStream_t s) {
-  void* ws = workspace(panel::ws_bytes(), 3);
-  const panel::WS w = panel::carve(ws);
-  int h = 0;
-  ZH_CHECK(hipMemcpyAsync(&h, w.err, sizeof(int), hipMemcpyDeviceToHost, s));
-  ZH_CHECK(hipStreamSynchronize(s));
-  return h;
+__global__ void clear_flag_kernel(int* __restrict__ flag) { *flag = 0; }
+__global__ void report_flag_kernel(const int* __restrict__ flag, int* __restrict__ info) {
+  if (*flag) *info = kErrSync;
+}
+
+void clear_leaf_error(hipStream_t s) {
+  void* ws = nullptr;
+  leaf_state(s, &ws);
+  hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(1), 0, s, panel::carve(ws).err);
+  ZH_LAUNCH_CHECK();
+}
+
+void report_leaf_error(int* info, hipStream_t s) {
+  void* ws = nullptr;
+  leaf_state(s, &ws);
+  hipLaunchKernelGGL(report_flag_kernel, dim3(1), dim3(1), 0, s, (const int*)panel::carve(ws).err, info);
+  ZH_LAUNCH_CHECK();
 }
 
 }  // namespace zmlhip

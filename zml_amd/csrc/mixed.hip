@@ -617,22 +617,26 @@ __global__ void scal_copy_f64_kernel(int64_t n, double a, const double* __restri
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) y[i] = a * x[i];
 }
-// partial[b] = sum over the block of x_i y_i (mode 0) or max |x_i| (mode 1)
+// max(a, b) that keeps a NaN from either side (fmax would drop it), so that a
+// non-finite residual or solution is never read as a small backward error
+__device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// partial[b] = sum over the block of x_i y_i (mode 0) or max |x_i| (mode 1, NaN kept)
 __global__ void reduce_f64_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                   int mode, double* __restrict__ partial) {
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    acc = mode == 0 ? acc + x[i] * y[i] : fmax(acc, fabs(x[i]));
+    acc = mode == 0 ? acc + x[i] * y[i] : nan_max(acc, fabs(x[i]));
   for (int off = 32; off > 0; off >>= 1) {
     const double o = __shfl_xor(acc, off);
-    acc = mode == 0 ? acc + o : fmax(acc, o);
+    acc = mode == 0 ? acc + o : nan_max(acc, o);
   }
   __shared__ double sw[4];
   if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = sw[0];
-    for (int w = 1; w < 4; ++w) t = mode == 0 ? t + sw[w] : fmax(t, sw[w]);
+    for (int w = 1; w < 4; ++w) t = mode == 0 ? t + sw[w] : nan_max(t, sw[w]);
     partial[blockIdx.x] = t;
   }
 }
@@ -1191,7 +1195,7 @@ double reduce_host(const Solver& S, int64_t n, const double* x, const double* y,
   ZH_CHECK(hipMemcpyAsync(h.data(), S.part, sizeof(double) * nb, hipMemcpyDeviceToHost, S.s));
   ZH_CHECK(hipStreamSynchronize(S.s));
   double acc = 0.0;
-  for (double v : h) acc = mode == 0 ? acc + v : std::max(acc, v);
+  for (double v : h) acc = mode == 0 ? acc + v : ((v > acc || v != v) ? v : acc);   // NaN kept
   return acc;
 }
 
@@ -1240,7 +1244,7 @@ void check_solve(const Solver& S) {
   int e = 0;
   ZH_CHECK(hipMemcpyAsync(&e, S.serr, sizeof(int), hipMemcpyDeviceToHost, S.s));
   ZH_CHECK(hipStreamSynchronize(S.s));
-  if (e) throw HipError(hipErrorLaunchTimeOut, "triangular solve stalled (spin limit reached)");
+  if (e) throw SyncError("triangular solve stalled (spin limit reached)");
 }
 
 double backward_error(const Solver& S, const double* r, const double* x) {
@@ -1717,14 +1721,15 @@ void skinny_gemm_f64(const Solver& S, int nr, double alpha, const double* A, int
   ZH_LAUNCH_CHECK();
 }
 
-// out[q] = max_i |M[q ld + i]| (out zeroed by the caller)
+// out[q] = max_i |M[q ld + i]| (out zeroed by the caller). A NaN is kept: its
+// bit pattern (sign cleared) orders above +inf in the integer maximum.
 __global__ void colmax_kernel(int64_t n, const double* __restrict__ M, int64_t ld, double* __restrict__ out) {
   const int q = blockIdx.y;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    acc = fmax(acc, fabs(M[q * ld + i]));
-  for (int off = 32; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off));
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&out[q], acc);
+    acc = nan_max(acc, fabs(M[q * ld + i]));
+  for (int off = 32; off > 0; off >>= 1) acc = nan_max(acc, __shfl_xor(acc, off));
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&out[q], fabs(acc));
 }
 
 struct CholWork {
@@ -2019,7 +2024,9 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   int info = 0;
   ZH_CHECK(hipMemcpyAsync(&info, S.info, sizeof(int), hipMemcpyDeviceToHost, s));
   ZH_CHECK(hipStreamSynchronize(s));
+  if (info == kErrSync) throw SyncError("dgesv_mixed: a panel's cross-workgroup wait gave up");
   res->info = info;
+  int ret = 0;   // first zero pivot of an FP64 fallback factorization (A singular)
   res->panels = tl_mix.panels;
   res->panels_optimistic = tl_mix.panels_fast;
 
@@ -2102,6 +2109,11 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
       ZH_CHECK(hipMemsetAsync(inf64, 0, sizeof(int), s));
       // A64 read row-major is A (row-major input) or A^T (column-major input)
       getrf_f64_rm(n, n, A64, n, ip64, inf64, s);
+      int h64 = 0;
+      ZH_CHECK(hipMemcpyAsync(&h64, inf64, sizeof(int), hipMemcpyDeviceToHost, s));
+      ZH_CHECK(hipStreamSynchronize(s));
+      if (h64 == kErrSync) throw SyncError("dgesv_mixed: a panel's cross-workgroup wait gave up");
+      if (h64 > 0 && ret == 0) ret = h64;
       ZH_CHECK(hipMemcpyAsync(xj, bj, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
       // x is an n x 1 row-major matrix (ld 1); a row-major (Left, uplo) solve is the
       // column-major (Right, inverted uplo) one on the same memory (trsm.zig:48-62)
@@ -2117,13 +2129,13 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
       residual(S, bj, xj, rj);
       be = backward_error(S, rj, xj);
     }
-    worst = std::max(worst, be);
+    if (be > worst || be != be) worst = be;   // a NaN backward error stays visible
     hipLaunchKernelGGL(store_f64_kernel, dim3(blocks(n)), dim3(256), 0, s, n, xj, xcol, xstride);
     ZH_LAUNCH_CHECK();
   }
   ZH_CHECK(hipStreamSynchronize(s));
   res->backward_error = worst;
-  return info;
+  return ret;
 }
 
 
@@ -2242,7 +2254,7 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
       if (ib[b]) first = (int)(b * NB_MIX + ib[b]);
     if (attempt == 0) res->info = first;
     if (!first) break;
-    if (attempt >= 4) throw std::runtime_error("dposv_mixed: the shifted low-precision Cholesky still fails");
+    if (attempt >= 4) return res->info;   // not positive definite: LAPACK-style info > 0, X untouched
     shift = shift == 0.0 ? 0x1p-10 : shift * 16.0;
     res->fallback = std::max(res->fallback, 1);
   }
@@ -2255,6 +2267,7 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
   double* Rc = vec + 2 * (size_t)n * MAXR;
   double* gw = nullptr;
   double worst = 0.0;
+  int ret = 0;   // first non-positive pivot of an FP64 fallback factorization
   std::vector<double> hm(2 * MAXR);
   for (int64_t q0 = 0; q0 < nrhs; q0 += MAXR) {
     const int nr = (int)std::min<int64_t>(MAXR, nrhs - q0);
@@ -2326,6 +2339,10 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
         ZH_CHECK(hipMemcpyAsync(A64, S64, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
         ZH_CHECK(hipMemsetAsync(inf64, 0, sizeof(int), s));
         potrf_f64_cm(kLower, n, A64, n, inf64, s);
+        int h64 = 0;
+        ZH_CHECK(hipMemcpyAsync(&h64, inf64, sizeof(int), hipMemcpyDeviceToHost, s));
+        ZH_CHECK(hipStreamSynchronize(s));
+        if (h64 > 0 && ret == 0) ret = h64;
         ZH_CHECK(hipMemcpyAsync(xq, bq, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         trsm_f64_cm(kLeft, kLower, kNoTrans, kNonUnit, n, 1, 1.0, A64, n, xq, n, s);
         trsm_f64_cm(kLeft, kLower, kTrans, kNonUnit, n, 1, 1.0, A64, n, xq, n, s);
@@ -2333,7 +2350,7 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
         residual(S, bq, xq, rq);
         be[q] = backward_error(S, rq, xq);
       }
-      worst = std::max(worst, be[q]);
+      if (be[q] > worst || be[q] != be[q]) worst = be[q];
       const int64_t col = q0 + q;
       const int64_t xstride = layout == kColMajor ? 1 : ldx;
       double* xcol = layout == kColMajor ? X + col * ldx : X + col;
@@ -2343,7 +2360,7 @@ int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, 
   }
   ZH_CHECK(hipStreamSynchronize(s));
   res->backward_error = worst;
-  return 0;
+  return ret;
 }
 
 }  // namespace zmlhip

@@ -138,6 +138,9 @@ __device__ __forceinline__ void update_row(T (&a)[W], T l, const T* __restrict__
 // ipiv[k] = piv_base + pivot row + 1 (1-based, global); *info gets
 // info_base + first zero pivot (1-based) when it is still 0.
 // MULTI: more than one workgroup (exchange through ws), else LDS only.
+// A wait that exceeds spin_limit polls (or any wait, when spin_limit < 0)
+// sets *ws.err and ends the kernel in every workgroup; the factors are then
+// invalid and the host reports the give-up.
 template <typename T, bool MULTI>
 __global__ void __launch_bounds__(pt_of<T>())
 leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restrict__ ipiv,
@@ -237,7 +240,9 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
         if (lane == 0) st64(rkrec, tagged(tag, s_knan ? 1u : 0u));
       }
       if (t == 0) st_val<T>(ws.val + ((size_t)slot * GMAX + g) * 2, tag, bi == INT64_MAX ? T(-1) : bv);
-      if (wv == 0) {
+      if (wv == 0 && spin_limit < 0) {   // give-up forced by the host (error-path tests)
+        if (lane == 0) { s_abort = 1; atomicExch(ws.err, 1); }
+      } else if (wv == 0) {
         long spins = 0;
         // (1) every workgroup's |a|
         T gv = T(-1);
@@ -285,7 +290,12 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
         }
       }
       __syncthreads();
-      if (s_abort) return;
+      if (s_abort) {
+        // gave up: the columns not reached keep their rows (identity pivots), so
+        // the interchanges the caller applies next stay inside the matrix
+        if (g == 0 && t < Wn) ipiv[t] = t < k ? s_piv[t] : (int32_t)(piv_base + t + 1);
+        return;
+      }
     }
 
     const int64_t p = s_p;

@@ -1,4 +1,4 @@
-// FP64 Cholesky (zml lapack/potrf.zig:17-240, potrf2.zig:17-217), column-major view.
+// FP64 / FP32 Cholesky (zml lapack/potrf.zig:17-240, potrf2.zig:17-217), column-major view.
 //
 // zml's potrf is left-looking with NB = 64; on the GPU the same factor is
 // computed right-looking (the per-element operation order of the lower,
@@ -35,14 +35,18 @@ constexpr int DB = 128;   // diagonal block of the driver
 // takes l_jj = sqrt(a_jj), r = fl(1 / l_jj) and l_ij = fl(a_ij * r) from it, so
 // the trailing update a_ic - l_ij * l_cj uses the same separately rounded
 // values, in the same right-looking order, as the unblocked algorithm.
+__device__ __forceinline__ double sqrt_rn(double x) { return sqrt(x); }   // IEEE, correctly rounded
+__device__ __forceinline__ float sqrt_rn(float x) { return sqrtf(x); }
+
+template <typename T>
 __global__ void __launch_bounds__(256)
-potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* __restrict__ info,
+potrf_diag_kernel(int nb, T* __restrict__ a, int64_t lda, int lower, int* __restrict__ info,
                   int64_t info_base) {
 #pragma clang fp contract(off)
-  __shared__ double colbuf[2][DB];
+  __shared__ T colbuf[2][DB];
   if (*info != 0) return;   // an earlier block already failed
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
-  double v[8][8];
+  T v[8][8];
   // every address of the nb x nb block is readable: all 36 loads are issued
   // unconditionally (clamped indices; the empty asm keeps the compiler from
   // sinking each one under its mask and waiting for it), then masked
@@ -60,7 +64,7 @@ potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* _
     for (int cc = 0; cc <= ii; ++cc) {
       asm volatile("" : "+v"(v[ii][cc]));
       const int r = tx + 16 * ii, c = ty + 16 * cc;
-      if (!(r < nb && c <= r)) v[ii][cc] = 0.0;
+      if (!(r < nb && c <= r)) v[ii][cc] = T(0);
     }
   // column 0 (ty == 0, cc == 0)
   if (ty == 0) {
@@ -71,12 +75,12 @@ potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* _
   __syncthreads();
   int fail = 0;
   for (int j = 0; j < nb; ++j) {
-    const double* cur = colbuf[j & 1];
-    const double d = cur[j];
-    if (!(d > 0.0)) { fail = j + 1; break; }   // non-positive or NaN pivot
-    const double ljj = sqrt(d);
-    const double rj = 1.0 / ljj;
-    double lr[8], lc[8];
+    const T* cur = colbuf[j & 1];
+    const T d = cur[j];
+    if (!(d > T(0))) { fail = j + 1; break; }   // non-positive or NaN pivot
+    const T ljj = sqrt_rn(d);
+    const T rj = T(1) / ljj;
+    T lr[8], lc[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       lr[k] = cur[tx + 16 * k] * rj;
@@ -130,7 +134,8 @@ potrf_diag_kernel(int nb, double* __restrict__ a, int64_t lda, int lower, int* _
 
 }  // namespace
 
-void potrf_f64_cm(int uplo, int64_t n, double* a, int64_t lda, int* info, hipStream_t s) {
+template <typename T>
+void potrf_cm(int uplo, int64_t n, T* a, int64_t lda, int* info, hipStream_t s) {
   if (n <= 0) return;
   const bool lower = uplo == kLower;
   constexpr int64_t OB = 512;   // block column of the trailing updates (their K)
@@ -139,26 +144,26 @@ void potrf_f64_cm(int uplo, int64_t n, double* a, int64_t lda, int* info, hipStr
   auto factor_block = [&](int64_t j0, int64_t jb, hipStream_t st) {
     for (int64_t k0 = j0; k0 < j0 + jb; k0 += DB) {
       const int64_t kb = std::min<int64_t>(DB, j0 + jb - k0);
-      double* d = at(k0, k0);
-      hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, (int)kb, d, lda, lower ? 1 : 0, info, k0);
+      T* d = at(k0, k0);
+      hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(256), 0, st, (int)kb, d, lda, lower ? 1 : 0, info, k0);
       ZH_LAUNCH_CHECK();
       const int64_t r = n - k0 - kb;           // rows below (lower) / columns right (upper)
       if (r <= 0) break;
       const int64_t w = j0 + jb - (k0 + kb);   // columns of the block column still to update
       if (lower) {
         // A21 <- A21 L11^-T ; the block column's part of A22 <- A22 - A21 A21^T
-        double* a21 = at(k0 + kb, k0);
-        trsm_f64_cm(kRight, kLower, kTrans, kNonUnit, r, kb, 1.0, d, lda, a21, lda, st);
+        T* a21 = at(k0 + kb, k0);
+        trsm_cm<T>(kRight, kLower, kTrans, kNonUnit, r, kb, T(1), d, lda, a21, lda, st);
         if (w > 0)
-          gemm_f64_cm(kNoTrans, kTrans, r, w, kb, -1.0, a21, lda, a21, lda, 1.0, at(k0 + kb, k0 + kb), lda, st,
-                      kLower);
+          gemm_cm<T>(kNoTrans, kTrans, r, w, kb, T(-1), a21, lda, a21, lda, T(1), at(k0 + kb, k0 + kb), lda, st,
+                     kLower);
       } else {
         // A12 <- U11^-T A12 ; the block row's part of A22 <- A22 - A12^T A12
-        double* a12 = at(k0, k0 + kb);
-        trsm_f64_cm(kLeft, kUpper, kTrans, kNonUnit, kb, r, 1.0, d, lda, a12, lda, st);
+        T* a12 = at(k0, k0 + kb);
+        trsm_cm<T>(kLeft, kUpper, kTrans, kNonUnit, kb, r, T(1), d, lda, a12, lda, st);
         if (w > 0)
-          gemm_f64_cm(kTrans, kNoTrans, w, r, kb, -1.0, a12, lda, a12, lda, 1.0, at(k0 + kb, k0 + kb), lda, st,
-                      kUpper);
+          gemm_cm<T>(kTrans, kNoTrans, w, r, kb, T(-1), a12, lda, a12, lda, T(1), at(k0 + kb, k0 + kb), lda, st,
+                     kUpper);
       }
     }
   };
@@ -185,21 +190,27 @@ void potrf_f64_cm(int uplo, int64_t n, double* a, int64_t lda, int* info, hipStr
     const int64_t c1 = j0 + jb, c2 = c1 + jb2;
     ZH_CHECK(hipStreamWaitEvent(s, ev_panel, 0));
     if (lower) {
-      const double* l21 = at(c1, j0);
-      gemm_f64_cm(kNoTrans, kTrans, r, jb2, jb, -1.0, l21, lda, l21, lda, 1.0, at(c1, c1), lda, s, kLower);
+      const T* l21 = at(c1, j0);
+      gemm_cm<T>(kNoTrans, kTrans, r, jb2, jb, T(-1), l21, lda, l21, lda, T(1), at(c1, c1), lda, s, kLower);
       ZH_CHECK(hipEventRecord(ev_next, s));
-      if (r2 > 0) syrk_f64_cm(kLower, kNoTrans, r2, jb, -1.0, at(c2, j0), lda, 1.0, at(c2, c2), lda, s);
+      if (r2 > 0) syrk_cm<T>(kLower, kNoTrans, r2, jb, T(-1), at(c2, j0), lda, T(1), at(c2, c2), lda, s);
     } else {
-      const double* u12 = at(j0, c1);
-      gemm_f64_cm(kTrans, kNoTrans, jb2, r, jb, -1.0, u12, lda, u12, lda, 1.0, at(c1, c1), lda, s, kUpper);
+      const T* u12 = at(j0, c1);
+      gemm_cm<T>(kTrans, kNoTrans, jb2, r, jb, T(-1), u12, lda, u12, lda, T(1), at(c1, c1), lda, s, kUpper);
       ZH_CHECK(hipEventRecord(ev_next, s));
-      if (r2 > 0) syrk_f64_cm(kUpper, kTrans, r2, jb, -1.0, at(j0, c2), lda, 1.0, at(c2, c2), lda, s);
+      if (r2 > 0) syrk_cm<T>(kUpper, kTrans, r2, jb, T(-1), at(j0, c2), lda, T(1), at(c2, c2), lda, s);
     }
     ZH_CHECK(hipStreamWaitEvent(sp, ev_next, 0));
     factor_block(c1, jb2, sp);
     ZH_CHECK(hipEventRecord(ev_panel, sp));
   }
   ZH_CHECK(hipStreamWaitEvent(s, ev_panel, 0));
+}
+template void potrf_cm<double>(int, int64_t, double*, int64_t, int*, hipStream_t);
+template void potrf_cm<float>(int, int64_t, float*, int64_t, int*, hipStream_t);
+
+void potrf_f64_cm(int uplo, int64_t n, double* a, int64_t lda, int* info, hipStream_t s) {
+  potrf_cm<double>(uplo, n, a, lda, info, s);
 }
 
 }  // namespace zmlhip

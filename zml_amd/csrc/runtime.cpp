@@ -3,6 +3,7 @@
 #include "zmlhip_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -28,6 +29,8 @@ struct DeviceState {
   int reserved_cus[kSparing] = {};
   hipStream_t reserved[kSparing] = {};   // every CU but the first reserved_cus[i]
   hipEvent_t ev[kEvents] = {};
+  hipEvent_t fence = nullptr;   // end of the previous call of this thread on this device
+  bool fence_set = false;
 };
 struct ThreadState {
   std::unordered_map<int, DeviceState> dev;
@@ -45,6 +48,7 @@ struct ThreadState {
         if (st) (void)hipStreamDestroy(st);
       for (auto& e : kv.second.ev)
         if (e) (void)hipEventDestroy(e);
+      if (kv.second.fence) (void)hipEventDestroy(kv.second.fence);
       (void)hipSetDevice(cur);
     }
   }
@@ -166,6 +170,24 @@ hipEvent_t aux_event(int idx) {
   if (!e) ZH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return e;
 }
+
+void call_begin(hipStream_t s) {
+  DeviceState& ds = current();
+  if (ds.fence_set) ZH_CHECK(hipStreamWaitEvent(s, ds.fence, 0));
+}
+
+void call_end(hipStream_t s) {
+  DeviceState& ds = current();
+  if (!ds.fence) ZH_CHECK(hipEventCreateWithFlags(&ds.fence, hipEventDisableTiming));
+  ZH_CHECK(hipEventRecord(ds.fence, s));
+  ds.fence_set = true;
+}
+
+namespace {
+std::atomic<long> g_panel_spin_limit{2000000L};
+}
+long panel_spin_limit() { return g_panel_spin_limit.load(std::memory_order_relaxed); }
+void set_panel_spin_limit(long polls) { g_panel_spin_limit.store(polls, std::memory_order_relaxed); }
 
 int device_cu_count() {
   int d = 0;

@@ -59,6 +59,10 @@ struct HipError : std::runtime_error {
   hipError_t code;
   HipError(hipError_t c, const char* what) : std::runtime_error(what), code(c) {}
 };
+// A bounded device-side wait gave up (kErrSync at the C boundary).
+struct SyncError : std::runtime_error {
+  explicit SyncError(const char* what) : std::runtime_error(what) {}
+};
 
 #define ZH_CHECK(expr)                                                                      \
   do {                                                                                      \
@@ -104,9 +108,40 @@ void profile_collect(int kind, double* ms, long long* launches, double* flops, d
 // Number of compute units of the current device.
 int device_cu_count();
 
+// Ordering of calls that share this thread's workspaces, internal streams and
+// events: call_begin makes `s` wait for the end of the previous call (whatever
+// stream it ran on), call_end marks the end of this one on `s`. Every device
+// entry point brackets its work with them, so two calls queued on two streams
+// never use the same scratch memory at the same time.
+void call_begin(hipStream_t s);
+void call_end(hipStream_t s);
+
+// Spin limit of the LU panel's cross-workgroup waits (polls before giving up).
+// A negative value makes every multi-workgroup panel give up at its first
+// exchange, which tests use to exercise the error path with kernels that all
+// exit normally.
+long panel_spin_limit();
+void set_panel_spin_limit(long polls);
+
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // ---- kernels exposed between translation units -----------------------------
+
+// Element-type generic forms (T = double or float) of the column-major
+// kernels below; the _f64 names are their FP64 instances.
+template <typename T>
+void gemm_cm(int transa, int transb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda,
+             const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s, int tri);
+template <typename T>
+void scale_cm(int64_t m, int64_t n, T beta, T* c, int64_t ldc, hipStream_t s, int tri);
+template <typename T>
+void trsm_cm(int side, int uplo, int transa, int diag, int64_t m, int64_t n, T alpha, const T* a,
+             int64_t lda, T* b, int64_t ldb, hipStream_t s);
+template <typename T>
+void syrk_cm(int uplo, int trans, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, T beta, T* c,
+             int64_t ldc, hipStream_t s);
+template <typename T>
+void potrf_cm(int uplo, int64_t n, T* a, int64_t lda, int* info, hipStream_t s);
 
 // Column-major FP64 GEMM: C = alpha * op(A) * op(B) + beta * C.
 // tri: 0 full, kLower / kUpper restrict the update to that triangle of C
@@ -142,19 +177,33 @@ void scal_f64(int64_t n, double alpha, double* x, int64_t incx, hipStream_t s);
 void swap_f64(int64_t n, double* x, int64_t incx, double* y, int64_t incy, hipStream_t s);
 void gemv_f64_cm(int trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
                  const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s);
+// x <- op(A)^-1 x, A n x n triangular (column-major view), x strided by incx.
+void trsv_f64_cm(int uplo, int trans, int diag, int64_t n, const double* a, int64_t lda, double* x,
+                 int64_t incx, hipStream_t s);
+// LAPACK dlange norms ('M', 'O'/'1', 'I', 'F'/'E') of an m x n column-major
+// matrix into *out (device); NaN entries propagate.
+void lange_f64_cm(char norm, int64_t m, int64_t n, const double* a, int64_t lda, double* out,
+                  hipStream_t s);
 
 // LAPACK drivers (getrf_f64.hip, potrf_f64.hip).
 void getrf_f64_rm(int64_t m, int64_t n, double* a, int64_t lda, int32_t* ipiv, int* info,
                   hipStream_t s);
-int leaf_error_flag(hipStream_t s);
+template <typename T>
+void getrf_rm(int64_t m, int64_t n, T* a, int64_t lda, int32_t* ipiv, int* info, hipStream_t s);
+// The LU panel's give-up flag: cleared at the start of every factorization;
+// report_leaf_error writes kErrSync into the device info when it is set.
+void clear_leaf_error(hipStream_t s);
+void report_leaf_error(int* info, hipStream_t s);
 void potrf_f64_cm(int uplo, int64_t n, double* a, int64_t lda, int* info, hipStream_t s);
 
 // In-place transpose of the leading n x n block of a matrix with leading
 // dimension ld (switches between row- and column-major storage).
-void transpose_square_inplace(int64_t n, double* a, int64_t ld, hipStream_t s);
+template <typename T>
+void transpose_square_inplace(int64_t n, T* a, int64_t ld, hipStream_t s);
 // Out-of-place: dst (cols x rows, ldd) = src^T (src rows x cols, lds; row-major view).
-void transpose_copy(int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst,
-                    int64_t ldd, hipStream_t s);
+template <typename T>
+void transpose_copy(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd,
+                    hipStream_t s);
 
 // Batched small solves (batched.hip)
 struct BatchPlan;
