@@ -27,6 +27,12 @@ systems/s and TFLOP/s (2/3 n^3 general, 1/3 n^3 SPD) next to the fraction of
 HBM bandwidth, with zml's gesv/posv (the oracle's C restatement) on all host
 cores as the CPU baseline.
 
+`--workload plain` measures the plain FP64 configuration (default n = 16384):
+zmlhip_dgetrf + zmlhip_dgetrs on A ~ U[-1, 1] (2/3 n^3), zmlhip_dpotrf +
+zmlhip_dpotrs on A = B B^T / n + I (1/3 n^3), one right-hand side each, and
+zmlhip_dgemm n x n x n (2 n^3), each step checked (backward error <= 16),
+with the FP64 trailing-update GEMM inside the LU as the roofline kernel.
+
 `--workload cholesky` measures the mixed-precision Cholesky configuration
 instead: A = B B^T / n + mu I (B with N(0, 1) entries), 16 right-hand sides,
 zmlhip_dposv_mixed against the plain FP64 zmlhip_dpotrf + zmlhip_dpotrs, at
@@ -52,25 +58,63 @@ PREC = {"fp32": 0, "fp16": 1, "bf16": 2}
 PROF_GEMM_F64, PROF_GEMM_LOWP = 1, 2
 
 
-def cpu_baseline(n=4096, reps=1):
-    """zml's getrf + getrs (oracle/zml_oracle.c restatement), one core."""
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name"), from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sizes=(1024, 2048, 4096), reps=(1, 1, 2), n_target=32768):
+    """zml's getrf + getrs (oracle/zml_oracle.c restatement), one core.
+
+    Times the reference's own small configuration (n = 1024) and n = 2048,
+    4096 on uniform[-1, 1] row-major systems, checks every solve's normwise
+    backward error (<= 16, u = 2^-53), fits t = a n^3 through the two largest
+    sizes and labels the n = 32768 time that fit gives "extrapolated" (it is
+    not run: the restatement needs most of an hour there). `value` is the rate
+    measured at the largest size."""
     import numpy as np
     import oracle as O
     O.lib()
-    best = None
-    for _ in range(reps):
-        a = O.fill_uniform(O.ROW, n, n, 42)
-        b = O.fill_uniform(O.ROW, n, 1, 43)
-        ip = np.zeros(n, np.int32)
-        t0 = time.perf_counter()
-        assert O.getrf(O.ROW, n, n, a, n, ip) == 0
-        O.getrs(O.ROW, 111, n, 1, a, n, ip, b, 1)
-        dt = time.perf_counter() - t0
-        best = dt if best is None else min(best, dt)
-    return {"value": round((2.0 / 3.0) * n ** 3 / best / 1e12, 6), "unit": "TFLOP/s", "cores": 1,
-            "kind": "port",
-            "sample": f"zml getrf+getrs restated in C (oracle/zml_oracle.c, NB=64, no FMA), "
-                      f"n={n} row-major uniform[-1,1], 1 RHS, best of {reps}, {best:.2f} s"}
+    times, bes = {}, {}
+    for n, r in zip(sizes, reps):
+        best = None
+        for _ in range(r):
+            a = O.fill_uniform(O.ROW, n, n, 42)
+            b = O.fill_uniform(O.ROW, n, 1, 43)
+            a0, b0 = a.copy(), b.copy()
+            ip = np.zeros(n, np.int32)
+            t0 = time.perf_counter()
+            assert O.getrf(O.ROW, n, n, a, n, ip) == 0
+            O.getrs(O.ROW, 111, n, 1, a, n, ip, b, 1)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        x = b.reshape(n)
+        res = b0.reshape(n) - a0.reshape(n, n) @ x
+        be = float(np.abs(res).max() / (np.abs(a0.reshape(n, n)).sum(1).max() * np.abs(x).max() * n * 2.0 ** -53))
+        assert be <= 16, f"CPU baseline n={n}: backward error {be}"
+        times[n], bes[n] = best, be
+    n1, n2 = sizes[-2], sizes[-1]
+    coef = (times[n1] / n1 ** 3 + times[n2] / n2 ** 3) / 2.0
+    n = sizes[-1]
+    return {"value": round((2.0 / 3.0) * n ** 3 / times[n] / 1e12, 6), "unit": "TFLOP/s", "cores": 1,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"zml getrf+getrs restated in C (oracle/zml_oracle.c, NB=64, no FMA), row-major "
+                      f"uniform[-1,1], 1 RHS, one core: value = n={n} (best of {reps[-1]}); every solve's "
+                      f"backward error <= 16",
+            "seconds": {str(k): round(v, 4) for k, v in times.items()},
+            "backward_error": {str(k): round(v, 3) for k, v in bes.items()},
+            "config_n1024_s": round(times[sizes[0]], 4),
+            "fit": {"model": "t = a n^3", "from_n": [n1, n2], "a_s": coef,
+                    f"n{n_target}_s": round(coef * n_target ** 3, 1),
+                    f"n{n_target}_tflops": round((2.0 / 3.0) * n_target ** 3 / (coef * n_target ** 3) / 1e12, 6),
+                    "label": "extrapolated"}}
 
 
 HBM_PEAK_GBPS = 8000.0
@@ -116,12 +160,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--n", type=int, default=None, help="order (32768; 16384 for --workload plain)")
     ap.add_argument("--delta", type=float, default=0.1)
     ap.add_argument("--precision", default="fp16", choices=sorted(PREC))
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="lu", choices=["lu", "cholesky", "batched"])
+    ap.add_argument("--workload", default="lu", choices=["lu", "cholesky", "batched", "plain"])
     ap.add_argument("--systems", type=int, default=20000)
     ap.add_argument("--mu", type=float, default=1e-2)
     ap.add_argument("--nrhs", type=int, default=16)
@@ -132,6 +176,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.n is None:
+        args.n = 16384 if args.workload == "plain" else 32768
+    if args.workload == "plain":
+        return bench_plain(args)
     if args.workload == "cholesky":
         return bench_cholesky(args)
     if args.workload == "batched":
@@ -256,6 +304,127 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+def bench_plain(args):
+    import torch
+    from zml_amd import _lib
+    zh = _lib.load()
+    dev = torch.device("cuda")
+    sh = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    n, ROW, COL = args.n, 101, 102
+    g = torch.Generator(device=dev)
+    g.manual_seed(42)
+    A = torch.rand(n, n, dtype=torch.float64, device=dev, generator=g).mul_(2).sub_(1)
+    b = torch.rand(n, 1, dtype=torch.float64, device=dev, generator=g).mul_(2).sub_(1)
+    Aw = torch.empty_like(A)
+    x = torch.empty_like(b)
+    ipiv = torch.zeros(n, dtype=torch.int32, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def be_of(M, xv, anorm):
+        res = b - M @ xv
+        return (res.abs().max() / (anorm * xv.abs().max() * n * 2.0 ** -53)).item()
+
+    def timed(step, steps, warmup):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            step()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        return ts[len(ts) // 2], ts
+
+    def lu_step():
+        Aw.copy_(A)
+        x.copy_(b)
+        info.zero_()
+        rc = zh.zmlhip_dgetrf(ROW, n, n, P(Aw), n, P(ipiv), P(info), sh)
+        rc |= zh.zmlhip_dgetrs(ROW, b"N", n, 1, P(Aw), n, P(ipiv), P(x), 1, sh)
+        if rc != 0:
+            raise RuntimeError(f"dgetrf/dgetrs: {_lib.last_error()}")
+
+    zh.zmlhip_profile_enable(1)
+    zh.zmlhip_profile_collect(PROF_GEMM_F64, *(ctypes.byref(v) for v in
+                              (ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double())))
+    for _ in range(args.warmup):
+        lu_step()
+    zh.zmlhip_profile_collect(PROF_GEMM_F64, *(ctypes.byref(v) for v in
+                              (ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double())))
+    t_lu, ts_lu = timed(lu_step, args.steps, 0)
+    zh.zmlhip_profile_enable(0)
+    ms, nl, fl, by = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+    zh.zmlhip_profile_collect(PROF_GEMM_F64, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(fl),
+                              ctypes.byref(by))
+    assert int(info.item()) == 0
+    be_lu = be_of(A, x, A.abs().sum(1).max().item())
+
+    # SPD: A = B B^T / n + I (B ~ N(0, 1)), column-major lower
+    Bm = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+    S = Bm @ Bm.T
+    del Bm
+    S.div_(n)
+    S.diagonal().add_(1.0)
+    S = 0.5 * (S + S.T)
+
+    def chol_step():
+        Aw.copy_(S)
+        x.copy_(b)
+        info.zero_()
+        rc = zh.zmlhip_dpotrf(COL, b"L", n, P(Aw), n, P(info), sh)
+        rc |= zh.zmlhip_dpotrs(COL, b"L", n, 1, P(Aw), n, P(x), n, sh)
+        if rc != 0:
+            raise RuntimeError(f"dpotrf/dpotrs: {_lib.last_error()}")
+
+    t_ch, ts_ch = timed(chol_step, args.steps, args.warmup)
+    assert int(info.item()) == 0
+    be_ch = be_of(S, x, S.abs().sum(1).max().item())
+    del S
+
+    # dgemm n x n x n: C = A B (row-major), B = A^T's storage reused as a second operand
+    Cm = torch.empty_like(A)
+
+    def gemm_step():
+        rc = zh.zmlhip_dgemm(ROW, 111, 111, n, n, n, 1.0, P(A), n, P(Aw), n, 0.0, P(Cm), n, sh)
+        if rc != 0:
+            raise RuntimeError(f"dgemm: {_lib.last_error()}")
+
+    Aw.copy_(A.T)
+    t_gm, ts_gm = timed(gemm_step, args.steps, args.warmup)
+    ref = A[:64].to(torch.float64) @ Aw
+    gemm_err = ((Cm[:64] - ref).abs().max() / ref.abs().max()).item()
+    del Cm, Aw
+
+    f_lu, f_ch, f_gm = (2.0 / 3.0) * n ** 3, n ** 3 / 3.0, 2.0 * n ** 3
+    valid = be_lu <= 16 and be_ch <= 16
+    rate = lambda f, t: round(f / t / 1e12, 3)
+    out = {
+        "metric": "plain FP64 LU solve TFLOP/s (dgetrf + dgetrs)",
+        "value": rate(f_lu, t_lu) if valid else None,
+        "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_lu * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (device-generated A ~ U[-1, 1]; SPD B B^T / n + I; 1 RHS)",
+        "config": {"workload": f"plain FP64 dgetrf+dgetrs, dpotrf+dpotrs, dgemm, n={n}, row-major LU",
+                   "n": n, "nrhs": 1},
+        "timing": "median of the timed steps, each bracketed by device synchronisations",
+        "lu": {"ms": round(t_lu * 1e3, 2), "tflops": rate(f_lu, t_lu), "backward_error": round(be_lu, 4),
+               "frac_fp64_peak": round(f_lu / t_lu / 1e12 / PEAK["fp64"], 4),
+               "ms_all": [round(t * 1e3, 2) for t in ts_lu]},
+        "cholesky": {"ms": round(t_ch * 1e3, 2), "tflops": rate(f_ch, t_ch), "backward_error": round(be_ch, 4),
+                     "frac_fp64_peak": round(f_ch / t_ch / 1e12 / PEAK["fp64"], 4),
+                     "ms_all": [round(t * 1e3, 2) for t in ts_ch]},
+        "dgemm": {"ms": round(t_gm * 1e3, 2), "tflops": rate(f_gm, t_gm), "rel_err_first_rows": gemm_err,
+                  "frac_fp64_peak": round(f_gm / t_gm / 1e12 / PEAK["fp64"], 4)},
+        "roofline": roofline_of("gemm_f64 trailing update inside dgetrf", ms.value, nl.value, fl.value, by.value,
+                                PEAK["fp64"], args.steps, None),
+    }
     print(json.dumps(out))
 
 

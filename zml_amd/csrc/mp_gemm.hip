@@ -252,7 +252,16 @@ mp_gemm_h_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restri
 // (columns beyond n) read the last valid row, whose products land only in C
 // entries that are not stored. Operands and products are those of
 // mp_gemm_h_kernel, in the same order.
-constexpr int DL_STAGES = 4;
+// Defaults of the streamed kernel: ring stages, workgroups per CU the
+// register budget is fitted to, and whether C is requested before the K loop
+// or after it. Measured at m = n = 32256 and 16384, K = 512 (FP16): three
+// stages, three workgroups per CU and C read after the products move 3.03 and
+// 3.11 TB/s, against 2.84 and 2.92 TB/s for four stages, two workgroups and C
+// requested first (a wave's slab waits count C's requests too, so prefetching
+// C delays the first slab by C's HBM latency); four workgroups per CU (two
+// stages) fall to 1.6 TB/s.
+constexpr int DL_STAGES = 3, DL_OCC = 3;
+constexpr bool DL_PREFETCH_C = false;
 __device__ __forceinline__ int dl_swz(int r) { return (r >> 1) & 3; }
 
 template <int N>
@@ -262,14 +271,13 @@ __device__ __forceinline__ void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
 }
 
-template <typename H>
-__global__ void __launch_bounds__(NTH, 2)
+template <typename H, int S = DL_STAGES, int OCC = DL_OCC, bool PFC = DL_PREFETCH_C>
+__global__ void __launch_bounds__(NTH, OCC)
 mp_gemm_h_dl_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restrict__ A, int64_t lda,
                     const H* __restrict__ BT, int64_t ldb, float beta, const float* __restrict__ rs,
                     const float* __restrict__ cs, float* __restrict__ C, int64_t ldc, int lower,
                     int64_t tri_tiles) {
   typedef typename MfmaH<H>::vec V;
-  constexpr int S = DL_STAGES;
   constexpr int ROWB = 64;                 // bytes per LDS row: 32 halfs
   __shared__ __attribute__((aligned(1024))) unsigned char lds[S][2][MT * ROWB];
   const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
@@ -314,7 +322,7 @@ mp_gemm_h_dl_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __res
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
   f16v cv[2][2];
-  load_c(cv, m, n, beta, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
+  if (PFC) load_c(cv, m, n, beta, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
 
   const int nk = (int)(K / BKH);
 #pragma unroll
@@ -350,6 +358,7 @@ mp_gemm_h_dl_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __res
         for (int y = 0; y < 2; ++y) acc[x][y] = MfmaH<H>::mma(av[x], bv[y], acc[x][y]);
     }
   }
+  if (!PFC) load_c(cv, m, n, beta, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
   epilogue(acc, cv, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
 }
 

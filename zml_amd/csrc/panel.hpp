@@ -416,7 +416,7 @@ __global__ void __launch_bounds__(256)
 apply_perm_kernel(int64_t ncols, T* __restrict__ a, int64_t lda, const int32_t* __restrict__ pairs) {
   constexpr int CW = 16, RG = 256 / CW, RPT = PERM_MAX / RG;   // 64 moves per thread
   __shared__ int32_t mv[2 * PERM_MAX];                          // (destination, source) per move
-  const int n = pairs[0];
+  const int n = min(pairs[0], PERM_MAX);
   for (int q = threadIdx.x; q < 2 * n; q += 256) mv[q] = pairs[1 + q];
   __syncthreads();
   const int col = threadIdx.x % CW, q0 = threadIdx.x / CW;

@@ -19,7 +19,7 @@ struct Buf {
   void* ptr = nullptr;
   size_t bytes = 0;
 };
-constexpr int kSlots = 24;
+constexpr int kSlots = 32;
 constexpr int kAux = 4, kEvents = 12;
 constexpr int kSparing = 8;   // CU-sparing streams kept per device (distinct reservations)
 struct DeviceState {
@@ -112,6 +112,7 @@ void profile_collect(int kind, double* ms, long long* launches, double* flops, d
 const char* last_error() { return tl_last_error.c_str(); }
 
 void* workspace(size_t bytes, int slot) {
+  if (slot < 0 || slot >= kSlots) throw std::runtime_error("workspace: slot out of range");
   DeviceState& ds = current();
   Buf& b = ds.slot[slot];
   if (b.bytes < bytes) {
