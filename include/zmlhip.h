@@ -72,6 +72,19 @@ int zmlhip_dswap(int n, double *x, int incx, double *y, int incy, void *stream);
 int zmlhip_dgemv(int layout, int transa, int m, int n, double alpha, const double *a, int lda,
                  const double *x, int incx, double beta, double *y, int incy, void *stream);
 
+/* zmlhip_dgemv without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that refine
+ * solutions, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. Same arguments and results as zmlhip_dgemv (the same kernels;
+ * bit-identical results while m * splits fits), but the partial sums of the
+ * no-transpose form live in a static per-device array of 2^20 doubles
+ * (8 MiB, held by each device the library runs on until the process ends)
+ * instead of a library workspace. Returns ZMLHIP_ENOTSUP, with nothing
+ * queued, when the no-transpose view (column-major, or the transposed view of
+ * a row-major call) has more than 2^20 rows. Calls that overlap on one
+ * device share that array and race; order them on one stream. */
+int zmlhip_dgemv_noalloc(int layout, int transa, int m, int n, double alpha, const double *a, int lda,
+                         const double *x, int incx, double beta, double *y, int incy, void *stream);
 /* cblas_dtrsv (src/cblas.zig:512; blas/trsv.zig:16-587) */
 int zmlhip_dtrsv(int layout, int uplo, int transa, int diag, int n, const double *a, int lda, double *x,
                  int incx, void *stream);

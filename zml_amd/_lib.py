@@ -66,6 +66,8 @@ SIGNATURES = {
     "zmlhip_dswap": (_c_int, [_c_int, _p, _c_int, _p, _c_int, _p]),
     "zmlhip_dgemv": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_double, _p, _c_int, _p, _c_int,
                               _c_double, _p, _c_int, _p]),
+    "zmlhip_dgemv_noalloc": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_double, _p, _c_int, _p, _c_int,
+                                      _c_double, _p, _c_int, _p]),
     "zmlhip_dtrsv": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _p, _c_int, _p, _c_int, _p]),
     "zmlhip_sgemm": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_float, _p, _c_int,
                               _p, _c_int, _c_float, _p, _c_int, _p]),

@@ -12,6 +12,7 @@
 #include "zmlhip_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace zmlhip {
 namespace {
@@ -581,8 +582,18 @@ __global__ void gemv_t_kernel(int64_t m, int64_t n, double alpha, const double* 
 
 }  // namespace
 
-void gemv_f64_cm(int trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
-                 const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s) {
+// Column splits of the no-transpose GEMV: about 2048 workgroups' worth of
+// partial sums (its scratch is nsplit x m doubles).
+static int gemv_splits(int64_t m, int64_t n) {
+  const int64_t rb = (m + GV_SPLIT_ROWS - 1) / GV_SPLIT_ROWS;
+  const int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, (2048 + rb - 1) / rb));
+  return (int)std::min<int64_t>(nsplit, n);
+}
+
+// part: nsplit x m doubles of scratch (the library's workspace when null)
+static void gemv_f64_cm_with(int trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
+                             const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s,
+                             double* part, int nsplit) {
   if (m <= 0 || n <= 0 || (alpha == 0.0 && beta == 1.0)) return;
   const bool nt = is_notrans(trans);
   const int64_t leny = nt ? m : n;
@@ -594,10 +605,8 @@ void gemv_f64_cm(int trans, int64_t m, int64_t n, double alpha, const double* a,
   }
   if (nt) {
     const int64_t rb = (m + GV_SPLIT_ROWS - 1) / GV_SPLIT_ROWS;
-    int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, (2048 + rb - 1) / rb));
-    nsplit = (int)std::min<int64_t>(nsplit, n);
     const int64_t cps = (n + nsplit - 1) / nsplit;
-    double* part = static_cast<double*>(workspace(sizeof(double) * nsplit * m, 6));
+    if (!part) part = static_cast<double*>(workspace(sizeof(double) * nsplit * m, 6));
     hipLaunchKernelGGL(gemv_n_partial, dim3((unsigned)rb, nsplit), dim3(GV_SPLIT_ROWS), 0, s, m, n,
                        a, lda, x, incx, part, cps);
     ZH_LAUNCH_CHECK();
@@ -609,6 +618,53 @@ void gemv_f64_cm(int trans, int64_t m, int64_t n, double alpha, const double* a,
                        incx, beta, y, incy);
     ZH_LAUNCH_CHECK();
   }
+}
+
+void gemv_f64_cm(int trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
+                 const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s) {
+  gemv_f64_cm_with(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, s, nullptr, gemv_splits(m, n));
+}
+
+// ---- allocation-free GEMV ---------------------------------------------------------
+// The partial sums of the no-transpose form live in this file-scope device
+// array instead of a library workspace, so the entry point that uses it calls
+// no allocator: each device holds its own copy for the life of the process.
+// Its address is taken once per device (hipGetSymbolAddress) with that device
+// current; the array's name itself is only a host-side stand-in. Every call on
+// a device shares its array (calls that overlap on one device race); nothing in
+// it has to start at zero (each call writes every partial sum it reads).
+static __device__ __align__(256) double scratch[kGemvScratchDoubles];
+
+double* gemv_scratch_address() {
+  constexpr int kMaxDevices = 64;
+  static double* addr[kMaxDevices] = {};
+  static std::mutex mu;
+  int dev = 0;
+  ZH_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("gemv: device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    ZH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    addr[dev] = static_cast<double*>(p);
+  }
+  return addr[dev];
+}
+
+int gemv_f64_cm_noalloc(int trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
+                        const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s) {
+  if (m <= 0 || n <= 0 || (alpha == 0.0 && beta == 1.0)) return 0;
+  int nsplit = gemv_splits(m, n);
+  double* part = nullptr;
+  if (is_notrans(trans) && alpha != 0.0) {
+    // the splits the allocating path takes when they fit (same sums, same
+    // order), fewer when nsplit x m would exceed the array
+    if ((size_t)m > kGemvScratchDoubles) return -1;
+    nsplit = (int)std::min<int64_t>(nsplit, (int64_t)(kGemvScratchDoubles / (size_t)m));
+    part = gemv_scratch_address();
+  }
+  gemv_f64_cm_with(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, s, part, nsplit);
+  return 0;
 }
 
 }  // namespace zmlhip

@@ -434,6 +434,31 @@ int zmlhip_dgemv(int layout, int transa, int m, int n, double alpha, const doubl
   });
 }
 
+int zmlhip_dgemv_noalloc(int layout, int transa, int m, int n, double alpha, const double* a, int lda,
+                         const double* x, int incx, double beta, double* y, int incy, void* stream) {
+  // no call fence: this entry point touches none of the thread's workspaces,
+  // streams or events (and creates none), only the device's static array
+  return guarded([&] {
+    if (!valid_layout(layout)) return -1;
+    if (!valid_trans(transa)) return -2;
+    int t = transa, mm = m, nn = n;
+    if (layout == kRowMajor) {
+      t = invert_trans(t);
+      std::swap(mm, nn);
+    }
+    if (mm < 0 || nn < 0) return -3;
+    if (lda < std::max(1, mm)) return -7;
+    if (incx == 0) return -9;
+    if (incy == 0) return -12;
+    if (gemv_f64_cm_noalloc(t, mm, nn, alpha, a, lda, x, incx, beta, y, incy, as_stream(stream)) != 0) {
+      set_last_error("dgemv_noalloc: more rows in the no-transpose view than the static scratch holds "
+                     "(1048576)");
+      return ZMLHIP_ENOTSUP;
+    }
+    return 0;
+  });
+}
+
 int zmlhip_dtrsv(int layout, int uplo, int transa, int diag, int n, const double* a, int lda, double* x,
                  int incx, void* stream) {
   return fenced(as_stream(stream), [&] {

@@ -177,6 +177,12 @@ void scal_f64(int64_t n, double alpha, double* x, int64_t incx, hipStream_t s);
 void swap_f64(int64_t n, double* x, int64_t incx, double* y, int64_t incy, hipStream_t s);
 void gemv_f64_cm(int trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
                  const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s);
+// The same GEMV with its partial sums in a static device array of
+// kGemvScratchDoubles doubles (no allocation); returns -1, with nothing
+// queued, when a no-transpose product has more than that many rows.
+constexpr size_t kGemvScratchDoubles = size_t(1) << 20;   // 8 MiB per device
+int gemv_f64_cm_noalloc(int trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
+                        const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s);
 // x <- op(A)^-1 x, A n x n triangular (column-major view), x strided by incx.
 void trsv_f64_cm(int uplo, int trans, int diag, int64_t n, const double* a, int64_t lda, double* x,
                  int64_t incx, hipStream_t s);

@@ -43,6 +43,9 @@ pub extern fn zmlhip_dgemm(layout: c_int, transa: c_int, transb: c_int, m: c_int
     stream: Stream) c_int;
 pub extern fn zmlhip_dgemv(layout: c_int, trans: c_int, m: c_int, n: c_int, alpha: f64, a: [*]const f64,
     lda: c_int, x: [*]const f64, incx: c_int, beta: f64, y: [*]f64, incy: c_int, stream: Stream) c_int;
+pub extern fn zmlhip_dgemv_noalloc(layout: c_int, trans: c_int, m: c_int, n: c_int, alpha: f64,
+    a: [*]const f64, lda: c_int, x: [*]const f64, incx: c_int, beta: f64, y: [*]f64, incy: c_int,
+    stream: Stream) c_int;
 pub extern fn zmlhip_dlange(layout: c_int, norm: u8, m: c_int, n: c_int, a: [*]const f64, lda: c_int,
     result: *f64, stream: Stream) c_int;
 
