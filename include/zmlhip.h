@@ -121,6 +121,12 @@ int zmlhip_spotrf(int layout, char uplo, int n, float *a, int lda, int *info, vo
  * ending normally: a test hook for the ZMLHIP_ESYNC path. Process-wide.
  * No counterpart in the reference. */
 int zmlhip_set_panel_spin_limit(long polls);
+/* FP64 LU block columns with more than `rows` rows (m - j0) are factored in
+ * one launch, the others leaf by leaf with GEMM updates in between; both give
+ * the same pivots. Default 16384 (the faster choice per height measured on
+ * MI355X); a negative value restores it, 0 makes every block column take the
+ * one-launch panel. Process-wide. No counterpart in the reference. */
+int zmlhip_set_panel_one_launch_rows(long rows);
 /* LAPACKE_dgetrs (src/linalg/lapack.zig:1575; lapack/getrs.zig:13-142) */
 int zmlhip_dgetrs(int layout, char trans, int n, int nrhs, const double *a, int lda,
                   const int *ipiv, double *b, int ldb, void *stream);

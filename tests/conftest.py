@@ -35,3 +35,13 @@ def torch_cuda():
     assert torch.cuda.is_available(), "GPU tests need the MI355X"
     torch.cuda.init()
     return torch
+
+
+@pytest.fixture(params=["by_height", "one_launch"])
+def lu_panel(request, zh):
+    """Runs an FP64 LU test under both panel strategies: the default choice by
+    block-column height (leaf by leaf up to 16384 rows, one launch above), and
+    every block column in one launch. Both must give zml's pivots."""
+    zh.zmlhip_set_panel_one_launch_rows(0 if request.param == "one_launch" else -1)
+    yield request.param
+    zh.zmlhip_set_panel_one_launch_rows(-1)

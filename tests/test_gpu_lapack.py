@@ -40,7 +40,7 @@ def _backward_error(A, x, b):
 
 
 @pytest.mark.parametrize("order", [ROW, COL])
-def test_config1_lu_n1024_pivots_and_solution(zh, order):
+def test_config1_lu_n1024_pivots_and_solution(zh, lu_panel, order):
     """Config 1: n = 1024, uniform[-1, 1] (seed 42), x_true uniform (seed 43), b = A x_true."""
     n = 1024
     a = O.fill_uniform(order, n, n, 42)
@@ -72,7 +72,7 @@ def test_config1_lu_n1024_pivots_and_solution(zh, order):
 @pytest.mark.parametrize("order", [ROW, COL])
 @pytest.mark.parametrize("m,n", [(1, 1), (3, 3), (64, 64), (65, 65), (300, 300), (513, 513),
                                  (700, 260), (260, 700), (1000, 999)])
-def test_dgetrf_shapes_match_oracle(zh, order, m, n):
+def test_dgetrf_shapes_match_oracle(zh, lu_panel, order, m, n):
     a = O.fill_uniform(order, m, n, 1000 + m * 7 + n)
     ld = (m if order == COL else n) + 5
     A = dense(order, m, n, a, m if order == COL else n)
@@ -88,7 +88,7 @@ def test_dgetrf_shapes_match_oracle(zh, order, m, n):
     assert np.abs(got - exp).max() <= 1e-11 * max(1.0, np.abs(exp).max())
 
 
-def test_dgetrf_singular_and_special(zh):
+def test_dgetrf_singular_and_special(zh, lu_panel):
     n = 200
     # identity: no interchanges, factors unchanged
     eye = np.eye(n).ravel()
@@ -118,7 +118,7 @@ def test_dgetrf_singular_and_special(zh):
 
 @pytest.mark.parametrize("order", [ROW, COL])
 @pytest.mark.parametrize("trans", [b"N", b"T"])
-def test_dgetrs_and_dgesv(zh, order, trans):
+def test_dgetrs_and_dgesv(zh, lu_panel, order, trans):
     n, nrhs = 333, 3
     a = O.fill_uniform(order, n, n, 77)
     A = dense(order, n, n, a, n)
@@ -235,7 +235,7 @@ def test_dpotrs_dposv(zh, order, uplo, n):
     assert np.abs(dense(order, n, nrhs, b2, ldb) - X).max() < 1e-11
 
 
-def test_lu_n16384_pivots_match_zml(zh):
+def test_lu_n16384_pivots_match_zml(zh, lu_panel):
     """Config 2 size: pivots of the 16384 x 16384 seeded matrix equal the
     golden vector written by tests/golden/make_lu_pivots.py (zml's getrf)."""
     path = os.path.join(GOLDEN, "lu_pivots_n16384_seed42.npz")

@@ -26,7 +26,7 @@ ESYNC = -1002
 
 @pytest.mark.parametrize("order", [ROW, COL])
 @pytest.mark.parametrize("m,n", [(1, 1), (5, 3), (64, 64), (300, 300), (700, 260), (260, 700)])
-def test_dgetrf2_matches_oracle_getrf2(zh, order, m, n):
+def test_dgetrf2_matches_oracle_getrf2(zh, lu_panel, order, m, n):
     a = O.fill_uniform(order, m, n, 500 + m + 3 * n)
     ld = m if order == COL else n
     ref = a.copy()
@@ -66,7 +66,7 @@ def test_dlamch_matches_oracle_for_every_character(zh):
 
 
 @pytest.mark.parametrize("order", [ROW, COL])
-def test_subnormal_pivot_divides_on_the_gpu(zh, order):
+def test_subnormal_pivot_divides_on_the_gpu(zh, lu_panel, order):
     """p = 2^-1050 < sfmin: 1/p overflows, so only division gives l = 2^-10."""
     p, q = 2.0 ** -1050, 2.0 ** -1060
     A = np.array([[p, 1.0, 2.0], [q, 5.0, 0.5], [0.0, 3.0, 1.0]])
@@ -78,7 +78,7 @@ def test_subnormal_pivot_divides_on_the_gpu(zh, order):
 
 
 @pytest.mark.parametrize("order", [ROW, COL])
-def test_subnormal_pivot_column_in_a_multi_workgroup_panel(zh, order):
+def test_subnormal_pivot_column_in_a_multi_workgroup_panel(zh, lu_panel, order):
     """A column of subnormals in the middle of a tall matrix (several panel
     workgroups): the GPU leaf divides exactly where zml's getrf2 does, so the
     factors equal the oracle's there and the pivots are identical."""
@@ -162,7 +162,7 @@ def test_spotrf_matches_f32_oracle(zh, order, uplo, n):
 
 
 # ---- the panel's bounded wait giving up ---------------------------------------------
-def test_panel_give_up_is_reported_once_and_cleared(zh, torch_cuda):
+def test_panel_give_up_is_reported_once_and_cleared(zh, lu_panel, torch_cuda):
     torch = torch_cuda
     n = 900   # four 256-row panel workgroups: the exchange is used
     a0 = O.fill_uniform(ROW, n, n, 77)
@@ -220,7 +220,7 @@ def test_mixed_solver_reports_a_panel_give_up(zh, torch_cuda):
 
 
 # ---- one thread, two streams ------------------------------------------------------
-def test_two_streams_do_not_share_scratch_concurrently(zh, torch_cuda):
+def test_two_streams_do_not_share_scratch_concurrently(zh, lu_panel, torch_cuda):
     torch = torch_cuda
     n1, n2 = 1500, 1100
     a1h, a2h = O.fill_uniform(ROW, n1, n1, 81), O.fill_uniform(ROW, n2, n2, 82)

@@ -504,6 +504,11 @@ int zmlhip_set_panel_spin_limit(long polls) {
   return 0;
 }
 
+int zmlhip_set_panel_one_launch_rows(long rows) {
+  set_panel_one_launch_rows(rows);
+  return 0;
+}
+
 int zmlhip_dlaswp(int layout, int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx,
                   void* stream) {
   return fenced(as_stream(stream), [&] { return dlaswp_dev(layout, n, a, lda, k1, k2, ipiv, incx, as_stream(stream)); });

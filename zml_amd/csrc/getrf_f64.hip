@@ -166,13 +166,14 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
   T* tmp = static_cast<T*>(workspace(sizeof(T) * nb * (size_t)std::max<int64_t>(n, 1), 1));
   // a panel in two parts: the optimistic attempt is queued first; its outcome
   // (and the full pivot search after a failed one) and the inverse follow
+  auto one_launch = [&](int64_t j0) { return ops.panel && m - j0 > ops.panel_min_rows; };
   auto panel_begin = [&](int64_t j0, int64_t jb) {
     return ops.try_panel && ops.try_panel(m, A, lda, j0, jb, ipiv, sp);
   };
   auto panel_end = [&](int64_t j0, int64_t jb, int64_t blk, bool attempted) {
     const bool ok = attempted && (!ops.try_panel_finish || ops.try_panel_finish(j0, jb, A, lda, sp));
     if (!ok) {
-      if (ops.panel) ops.panel(m, A, lda, j0, jb, ipiv, info, sp);
+      if (one_launch(j0)) ops.panel(m, A, lda, j0, jb, ipiv, info, sp);
       else factor_panel<T>(m, A, lda, j0, jb, ipiv, info, ops, sp, 2);
     }
     T* invp = ops.inv_store ? ops.inv_store(j0) : inv + (blk & 1) * nb * nb;
@@ -189,7 +190,7 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
     const int64_t jb = std::min(nb, mn - j0);
     const T* invb = ops.inv_store ? ops.inv_store(j0) : inv + (blk & 1) * nb * nb;
     ZH_CHECK(hipStreamWaitEvent(s, ev_panel, 0));
-    const int32_t* pr = ops.block_perm ? ops.block_perm(j0) : nullptr;
+    const int32_t* pr = ops.block_perm && (!ops.panel || one_launch(j0)) ? ops.block_perm(j0) : nullptr;
     if (!pr) pr = compose_perm((int)j0, (int)(j0 + jb - 1), ipiv, s, 4);
     const int64_t c1 = j0 + jb;                       // first column right of the block
     const int64_t jb2 = std::min(nb, std::max<int64_t>(0, mn - c1));   // next block width
@@ -337,13 +338,22 @@ static const int32_t* fused_block_perm(int64_t j0) {
 // FP64 factors each block column in one launch (panel_fused.hpp).
 template <typename T>
 void getrf_rm(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info, hipStream_t s) {
-  // The pivot panel keeps one workgroup per CU resident for the whole panel
-  // (512 rows each for FP64, panel_fused.hpp; 512 rows per leaf for FP32);
-  // when its first grid fits in a quarter of the CUs, those CUs are kept free
-  // of trailing updates so the panel never waits for GEMM tiles to drain.
-  // (rounded up to a power of two from 8, so few distinct masked streams exist)
-  const int rows_per_wg = sizeof(T) == 8 ? panel::FPT : panel::pt_of<T>();
-  const int leaf_wgs = (int)((m + rows_per_wg - 1) / rows_per_wg);
+  // The pivot panels keep one workgroup per CU resident for the whole panel:
+  // FP64 block columns taller than a height threshold are factored in one launch
+  // (panel_fused.hpp, 512 rows per workgroup), shorter ones (and FP32) leaf by
+  // leaf (256 / 512 rows per workgroup). When the first grid fits in a quarter
+  // of the CUs, those CUs are kept free of trailing updates so the panel never
+  // waits for GEMM tiles to drain (rounded up to a power of two from 8, so few
+  // distinct masked streams exist).
+  // The height threshold (panel_one_launch_rows, default 16384): measured per
+  // panel height on one MI355X, the one-launch panel wins above 16384 rows
+  // (where the leaf's 256-row grid no longer fits the reserved CUs) and loses
+  // below (n = 16384: 242 ms against 172 ms), its in-kernel update of the
+  // block column being slower than the MFMA GEMMs the leaf path launches
+  // between leaves.
+  const int64_t kFusedMinRows = panel_one_launch_rows();
+  const int64_t first_rows = sizeof(T) == 8 && m > kFusedMinRows ? panel::FPT : panel::pt_of<T>();
+  const int leaf_wgs = (int)((m + first_rows - 1) / first_rows);
   int spare = 0;
   if (leaf_wgs > 1) {
     spare = 8;
@@ -355,6 +365,7 @@ void getrf_rm(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info,
   if constexpr (sizeof(T) == 8) {
     ops.panel = fused_panel_f64;
     ops.block_perm = fused_block_perm;
+    ops.panel_min_rows = kFusedMinRows;
   }
   getrf_lookahead<T>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }

@@ -64,9 +64,12 @@ struct LuOps {
   bool (*try_panel_finish)(int64_t j0, int64_t jb, T* A, int64_t lda, hipStream_t s) = nullptr;
   // optional: factors block column [j0, j0 + jb) in one launch, leaves it in
   // LAPACK row order and provides its composed interchanges through
-  // block_perm (the leaf-by-leaf factor_panel when null)
+  // block_perm (the leaf-by-leaf factor_panel when null). It is used for the
+  // block columns with more than panel_min_rows rows (m - j0); the others
+  // take factor_panel, and their interchanges are composed by the driver.
   void (*panel)(int64_t m, T* A, int64_t lda, int64_t j0, int64_t jb, int32_t* ipiv, int* info,
                 hipStream_t s) = nullptr;
+  int64_t panel_min_rows = 0;
 };
 
 int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, int64_t lda,

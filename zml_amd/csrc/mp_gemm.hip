@@ -254,14 +254,14 @@ mp_gemm_h_kernel(int64_t m, int64_t n, int64_t K, float alpha, const H* __restri
 // mp_gemm_h_kernel, in the same order.
 // Defaults of the streamed kernel: ring stages, workgroups per CU the
 // register budget is fitted to, and whether C is requested before the K loop
-// or after it. Measured at m = n = 32256 and 16384, K = 512 (FP16): three
-// stages, three workgroups per CU and C read after the products move 3.03 and
-// 3.11 TB/s, against 2.84 and 2.92 TB/s for four stages, two workgroups and C
-// requested first (a wave's slab waits count C's requests too, so prefetching
-// C delays the first slab by C's HBM latency); four workgroups per CU (two
-// stages) fall to 1.6 TB/s.
-constexpr int DL_STAGES = 3, DL_OCC = 3;
-constexpr bool DL_PREFETCH_C = false;
+// or after it. Alone (m = n = 32256 and 16384, K = 512, FP16), three stages,
+// three workgroups per CU and C read after the products move 3.03 and 3.11
+// TB/s against 2.84 and 2.92 TB/s for four stages, two workgroups and C first;
+// but inside the mixed LU the third workgroup per CU takes the slots the panel
+// stream's kernels need, and the solve slows from 138 to 153 ms at n = 32768.
+// Four workgroups per CU (two stages) fall to 1.6 TB/s.
+constexpr int DL_STAGES = 4, DL_OCC = 2;
+constexpr bool DL_PREFETCH_C = true;
 __device__ __forceinline__ int dl_swz(int r) { return (r >> 1) & 3; }
 
 template <int N>

@@ -165,9 +165,17 @@ panel_fused_kernel(int64_t M, int Wd, T* __restrict__ P, int64_t ld, int32_t* __
       T wmax = v;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
-      int wpos = (v == wmax && v >= T(0)) ? pos : INT32_MAX;
+      // the lowest position among the lanes holding the maximum: usually one
+      // lane holds it, whose position is read directly
+      const uint64_t hit = __ballot(v == wmax && v >= T(0));
+      int wpos = INT32_MAX;
+      if (hit && (hit & (hit - 1)) == 0) {
+        wpos = __builtin_amdgcn_readlane(pos, (int)__builtin_ctzll(hit));
+      } else if (hit) {
+        wpos = (v == wmax && v >= T(0)) ? pos : INT32_MAX;
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) wpos = min(wpos, __shfl_xor(wpos, off));
+        for (int off = 32; off > 0; off >>= 1) wpos = min(wpos, __shfl_xor(wpos, off));
+      }
       if (wpos != INT32_MAX && pos == wpos && own) {
 #pragma unroll
         for (int c = 0; c < W; ++c) stage[wv][c] = a[c];
@@ -315,17 +323,45 @@ panel_fused_kernel(int64_t M, int Wd, T* __restrict__ P, int64_t ld, int32_t* __
     if (s_abort) goto give_up;
     {
       // L11: the pivot rows' leaf columns (unit lower part), in pivot order
-      for (int e = t; e < wn * wn; e += PT) {
-        const int q = e / wn, j = e % wn;
-        L11[q][j] = j < q ? P[s_prow[c0 + q] * ld + c0 + j] : T(0);
+      {
+        constexpr int PER_L = W * W / PT;
+        T v[PER_L];
+#pragma unroll
+        for (int r = 0; r < PER_L; ++r) {
+          const int e = t + r * PT, q = e / W, j = e % W;
+          v[r] = j < q ? P[s_prow[c0 + q] * ld + c0 + j] : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < PER_L; ++r) {
+          const int e = t + r * PT;
+          L11[e / W][e % W] = v[r];
+        }
       }
       const bool upd = own && pos >= r0;   // rows still unpivoted
+      // this row's next SUB columns of A22, requested one step ahead of their use
+      constexpr int SUB = 8;
+      T nxt[SUB];
+      auto load_next = [&](int col, int lim) {
+#pragma unroll
+        for (int c = 0; c < SUB; ++c) nxt[c] = (upd && col + c < lim) ? P[i * ld + col + c] : T(0);
+      };
+      load_next(r0, r0 + CH < Wd ? r0 + CH : Wd);
       for (int cc = r0; cc < Wd; cc += CH) {
         const int cw = Wd - cc < CH ? Wd - cc : CH;
         // A12 chunk of the pivot rows
-        for (int e = t; e < wn * CH; e += PT) {
-          const int q = e / CH, c = e % CH;
-          U12[q][c] = c < cw ? P[s_prow[c0 + q] * ld + cc + c] : T(0);
+        {
+          constexpr int PER_U = W * CH / PT;
+          T v[PER_U];
+#pragma unroll
+          for (int r = 0; r < PER_U; ++r) {
+            const int e = t + r * PT, q = e / CH, c = e % CH;
+            v[r] = c < cw ? P[s_prow[c0 + q] * ld + cc + c] : T(0);
+          }
+#pragma unroll
+          for (int r = 0; r < PER_U; ++r) {
+            const int e = t + r * PT;
+            U12[e / CH][e % CH] = v[r];
+          }
         }
         __syncthreads();
         // forward substitution, TPC threads per column (one wave holds a
@@ -349,21 +385,24 @@ panel_fused_kernel(int64_t M, int Wd, T* __restrict__ P, int64_t ld, int32_t* __
         // every unpivoted row: a_i -= l_i U12 (chunk), 16 columns at a time
         if (upd) {
 #pragma unroll
-          for (int c4 = 0; c4 < CH; c4 += 16) {
+          for (int c4 = 0; c4 < CH; c4 += SUB) {
             if (c4 < cw) {
-              T acc[16];
+              T acc[SUB];
 #pragma unroll
-              for (int c = 0; c < 16; ++c) acc[c] = (c4 + c < cw) ? P[i * ld + cc + c4 + c] : T(0);
+              for (int c = 0; c < SUB; ++c) acc[c] = nxt[c];
+              // the next SUB columns: the rest of this chunk, else the next chunk's first
+              if (c4 + SUB < cw) load_next(cc + c4 + SUB, cc + cw);
+              else load_next(cc + CH, cc + 2 * CH < Wd ? cc + 2 * CH : Wd);
 #pragma unroll
               for (int q = 0; q < W; ++q) {
                 if (q < wn) {
                   const T lq = a[q];
 #pragma unroll
-                  for (int c = 0; c < 16; ++c) acc[c] = fma(-lq, U12[q][c4 + c], acc[c]);
+                  for (int c = 0; c < SUB; ++c) acc[c] = fma(-lq, U12[q][c4 + c], acc[c]);
                 }
               }
 #pragma unroll
-              for (int c = 0; c < 16; ++c)
+              for (int c = 0; c < SUB; ++c)
                 if (c4 + c < cw) P[i * ld + cc + c4 + c] = acc[c];
             }
           }

@@ -190,6 +190,15 @@ std::atomic<long> g_panel_spin_limit{2000000L};
 long panel_spin_limit() { return g_panel_spin_limit.load(std::memory_order_relaxed); }
 void set_panel_spin_limit(long polls) { g_panel_spin_limit.store(polls, std::memory_order_relaxed); }
 
+namespace {
+constexpr long kOneLaunchRowsDefault = 16384;
+std::atomic<long> g_one_launch_rows{kOneLaunchRowsDefault};
+}
+long panel_one_launch_rows() { return g_one_launch_rows.load(std::memory_order_relaxed); }
+void set_panel_one_launch_rows(long rows) {
+  g_one_launch_rows.store(rows < 0 ? kOneLaunchRowsDefault : rows, std::memory_order_relaxed);
+}
+
 int device_cu_count() {
   int d = 0;
   ZH_CHECK(hipGetDevice(&d));

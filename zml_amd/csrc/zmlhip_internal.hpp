@@ -122,6 +122,9 @@ void call_end(hipStream_t s);
 // exit normally.
 long panel_spin_limit();
 void set_panel_spin_limit(long polls);
+// FP64 LU block columns with more rows than this take the one-launch panel.
+long panel_one_launch_rows();
+void set_panel_one_launch_rows(long rows);
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
