@@ -82,6 +82,56 @@ __device__ __forceinline__ void amax(V& v, I& i, V v2, I i2) {
   if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
 }
 
+// ---- wave-wide (value, index) maximum through DPP ---------------------------
+// The key: a value (|a| >= 0, or -1 for "no candidate"), an index that breaks
+// ties (lowest wins: a row, a position or a workgroup) and a payload carried
+// along with the winner. The reduction runs in registers: quad permutations,
+// row rotations by 4 and 8, then the row broadcasts 15 and 31 (CDNA DPP), so
+// lane 63 ends with the maximum over the wave, which every lane then reads.
+// Keys are totally ordered, so the order in which lanes meet does not matter.
+struct WaveKey {
+  double v;
+  int idx;
+  int pay;
+};
+__device__ __forceinline__ bool key_better(const WaveKey& a, const WaveKey& b) {
+  return a.v > b.v || (a.v == b.v && a.idx < b.idx);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ WaveKey key_dpp(const WaveKey& k) {
+  // lanes whose source is outside the pattern keep their own key
+  const uint64_t b = (uint64_t)__double_as_longlong(k.v);
+  const int hi = (int)(b >> 32), lo = (int)b;
+  WaveKey r;
+  const uint32_t rhi = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+  const uint32_t rlo = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+  r.v = __longlong_as_double((long long)(((uint64_t)rhi << 32) | rlo));
+  r.idx = __builtin_amdgcn_update_dpp(k.idx, k.idx, CTRL, ROW_MASK, 0xf, false);
+  r.pay = __builtin_amdgcn_update_dpp(k.pay, k.pay, CTRL, ROW_MASK, 0xf, false);
+  return r;
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ void key_step(WaveKey& k) {
+  const WaveKey o = key_dpp<CTRL, ROW_MASK>(k);
+  if (key_better(o, k)) k = o;
+}
+__device__ __forceinline__ WaveKey wave_best(WaveKey k) {
+  key_step<0xb1>(k);         // quad_perm [1,0,3,2]
+  key_step<0x4e>(k);         // quad_perm [2,3,0,1]
+  key_step<0x124>(k);        // row_ror:4
+  key_step<0x128>(k);        // row_ror:8  (every lane: its row's best)
+  key_step<0x142, 0xa>(k);   // row_bcast:15 into rows 1 and 3
+  key_step<0x143, 0xc>(k);   // row_bcast:31 into rows 2 and 3 (lane 63: the wave's best)
+  const uint64_t b = (uint64_t)__double_as_longlong(k.v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(b >> 32), 63);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)b, 63);
+  WaveKey r;
+  r.v = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+  r.idx = __builtin_amdgcn_readlane(k.idx, 63);
+  r.pay = __builtin_amdgcn_readlane(k.pay, 63);
+  return r;
+}
+
 // Rank-1 update of one row for column k: a[k] <- l, a[c] <- a[c] - l u[c]
 // for c > k, separately rounded. The register row is handled in groups of 8
 // columns so that every index is a compile-time constant; k is uniform, so
@@ -181,18 +231,17 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
       const T f = fabs(ak);
       if (f == f) v = f;
     }
-    T wmax = v;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
-    const uint64_t hit = __ballot(v == wmax && v >= T(0));
+    const WaveKey wk = wave_best(WaveKey{(double)v, lane, 0});
+    const T wmax = (T)wk.v;
+    const int wlane = wk.v >= 0.0 ? wk.idx : -1;   // lowest lane (= row) holding the maximum
     if (lane == 0) {
       s_v[wv] = wmax;
-      s_i[wv] = hit ? (int64_t)g * PT + wv * 64 + (__ffsll((unsigned long long)hit) - 1) : INT64_MAX;
+      s_i[wv] = wlane >= 0 ? (int64_t)g * PT + wv * 64 + wlane : INT64_MAX;
     }
     if (MULTI) {
       // stage the wave's candidate row and row k for publishing (the barrier
       // below orders these writes before the publishing wave reads them)
-      if (hit && lane == __ffsll((unsigned long long)hit) - 1) {
+      if (lane == wlane) {
 #pragma unroll
         for (int c = 0; c < W; ++c) stage[wv][c] = a[c];
       }
@@ -261,11 +310,10 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
           if (++spins > spin_limit) { if (lane == 0) { s_abort = 1; atomicExch(ws.err, 1); } break; }
           __builtin_amdgcn_s_sleep(1);
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const T v2 = __shfl_xor(gv, off);
-          const int g2 = __shfl_xor(gg, off);
-          amax(gv, gg, v2, g2);
+        {
+          const WaveKey bk = wave_best(WaveKey{(double)gv, gg, 0});
+          gv = (T)bk.v;
+          gg = bk.idx;
         }
         // (2) the winner's record and row k
         if (!s_abort) {
@@ -346,65 +394,88 @@ constexpr int PERM_MAX = 1024;  // pairs (2 x block of 512 interchanges)
 
 // Composes the interchanges i <-> ipiv[i]-1 for i = k1..k2 (ascending, every
 // ipiv[i]-1 >= i) into a list of moves dst <- src over the rows touched.
-// One wave: the lanes stage the pivots and clear an open-addressing table
-// (row -> slot) of the rows beyond the block, then one lane walks the
-// interchanges in order, each a table probe and a swap in LDS.
-// pairs[0] = count, then (dst, src) pairs.
+// One wave: the rows beyond the block enter an open-addressing table
+// (row -> slot) in parallel, each interchange's partner slot is looked up in
+// parallel, one lane then walks the interchanges in order (a swap of two LDS
+// words each), and the slots whose content changed become moves, compacted by
+// a ballot. pairs[0] = count, then (dst, src) pairs in slot order.
 constexpr int PERM_HT = 2 * PERM_MAX;   // power of two, load factor <= 1/4
 static_assert((PERM_HT & (PERM_HT - 1)) == 0, "table size must be a power of two");
-__global__ void compose_perm_kernel(int k1, int k2, const int32_t* __restrict__ ipiv,
-                                    int32_t* __restrict__ pairs) {
-  __shared__ int32_t pos[PERM_MAX], cont[PERM_MAX], piv[PERM_MAX / 2];
+__device__ __forceinline__ uint32_t perm_hash(int p) { return ((uint32_t)p * 2654435761u) & (PERM_HT - 1); }
+__global__ void __launch_bounds__(64)
+compose_perm_kernel(int k1, int k2, const int32_t* __restrict__ ipiv, int32_t* __restrict__ pairs) {
+  __shared__ int32_t pos[PERM_MAX], cont[PERM_MAX], partner[PERM_MAX / 2];
   __shared__ int32_t hkey[PERM_HT], hslot[PERM_HT];
   __shared__ int s_n;
   const int lane = threadIdx.x;
   const int kb = k2 - k1 + 1;
-  for (int q = lane; q < kb; q += 64) { pos[q] = k1 + q; cont[q] = k1 + q; piv[q] = ipiv[k1 + q] - 1; }
+  for (int q = lane; q < kb; q += 64) { pos[q] = k1 + q; cont[q] = k1 + q; }
   for (int q = lane; q < PERM_HT; q += 64) hkey[q] = -1;
+  if (lane == 0) s_n = kb;
+  __syncthreads();
+  // the distinct rows beyond the block enter the table
+  for (int q = lane; q < kb; q += 64) {
+    const int p = ipiv[k1 + q] - 1;
+    if (p > k2) {
+      uint32_t h = perm_hash(p);
+      for (;;) {
+        const int prev = atomicCAS(&hkey[h], -1, p);
+        if (prev == -1 || prev == p) break;
+        h = (h + 1) & (PERM_HT - 1);
+      }
+    }
+  }
+  __syncthreads();
+  // each of them gets a slot after the block's
+  for (int h = lane; h < PERM_HT; h += 64) {
+    const int key = hkey[h];
+    if (key >= 0) {
+      const int sl = atomicAdd(&s_n, 1);
+      hslot[h] = sl;
+      pos[sl] = key;
+      cont[sl] = key;
+    }
+  }
+  __syncthreads();
+  // the slot each interchange swaps with
+  for (int q = lane; q < kb; q += 64) {
+    const int p = ipiv[k1 + q] - 1;
+    int qp;
+    if (p <= k2) {
+      qp = p - k1;
+    } else {
+      uint32_t h = perm_hash(p);
+      while (hkey[h] != p) h = (h + 1) & (PERM_HT - 1);
+      qp = hslot[h];
+    }
+    partner[q] = qp;
+  }
   __syncthreads();
   if (lane == 0) {
-    int n = kb;
-    for (int i = 0; i < kb; ++i) {
-      const int p = piv[i];
-      int qp;
-      if (p <= k2) {
-        qp = p - k1;
-      } else {
-        uint32_t h = ((uint32_t)p * 2654435761u) & (PERM_HT - 1);
-        for (;;) {
-          const int key = hkey[h];
-          if (key == p) { qp = hslot[h]; break; }
-          if (key < 0) {
-            hkey[h] = p;
-            hslot[h] = n;
-            pos[n] = p;
-            cont[n] = p;
-            qp = n++;
-            break;
-          }
-          h = (h + 1) & (PERM_HT - 1);
-        }
-      }
-      if (qp != i) {
-        const int tmp = cont[i];
-        cont[i] = cont[qp];
+    for (int q = 0; q < kb; ++q) {
+      const int qp = partner[q];
+      if (qp != q) {
+        const int tmp = cont[q];
+        cont[q] = cont[qp];
         cont[qp] = tmp;
       }
     }
-    s_n = n;
   }
   __syncthreads();
   const int n = s_n;
-  if (lane == 0) {
-    int c = 0;
-    for (int q = 0; q < n; ++q)
-      if (cont[q] != pos[q]) {
-        pairs[1 + 2 * c] = pos[q];
-        pairs[2 + 2 * c] = cont[q];
-        ++c;
-      }
-    pairs[0] = c;
+  int c = 0;
+  for (int q0 = 0; q0 < n; q0 += 64) {
+    const int q = q0 + lane;
+    const bool moved = q < n && cont[q] != pos[q];
+    const uint64_t m = __ballot(moved);
+    if (moved) {
+      const int at = c + (int)__popcll(m & ((1ull << lane) - 1ull));
+      pairs[1 + 2 * at] = pos[q];
+      pairs[2 + 2 * at] = cont[q];
+    }
+    c += (int)__popcll(m);
   }
+  if (lane == 0) pairs[0] = c;
 }
 
 // Applies the moves to columns [0, ncols) of a row-major matrix: every

@@ -162,20 +162,10 @@ panel_fused_kernel(int64_t M, int Wd, T* __restrict__ P, int64_t ld, int32_t* __
         const T f = fabs(ak);
         if (f == f) v = f;
       }
-      T wmax = v;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
-      // the lowest position among the lanes holding the maximum: usually one
-      // lane holds it, whose position is read directly
-      const uint64_t hit = __ballot(v == wmax && v >= T(0));
-      int wpos = INT32_MAX;
-      if (hit && (hit & (hit - 1)) == 0) {
-        wpos = __builtin_amdgcn_readlane(pos, (int)__builtin_ctzll(hit));
-      } else if (hit) {
-        wpos = (v == wmax && v >= T(0)) ? pos : INT32_MAX;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) wpos = min(wpos, __shfl_xor(wpos, off));
-      }
+      // the wave's (largest |a|, lowest position)
+      const WaveKey wk = wave_best(WaveKey{(double)v, (own && pos >= k) ? pos : INT32_MAX, 0});
+      const T wmax = (T)wk.v;
+      const int wpos = wk.v >= 0.0 ? wk.idx : INT32_MAX;
       if (wpos != INT32_MAX && pos == wpos && own) {
 #pragma unroll
         for (int c = 0; c < W; ++c) stage[wv][c] = a[c];
@@ -242,11 +232,11 @@ panel_fused_kernel(int64_t M, int Wd, T* __restrict__ P, int64_t ld, int32_t* __
           if (++spins > spin_limit) { if (lane == 0) { s_abort = 1; atomicExch(ws.err, 1); } break; }
           __builtin_amdgcn_s_sleep(1);
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const T v2 = __shfl_xor(gv, off);
-          const int p2 = __shfl_xor(gp, off), g2 = __shfl_xor(gg, off);
-          if (better(v2, p2, gv, gp)) { gv = v2; gp = p2; gg = g2; }
+        {
+          const WaveKey bk = wave_best(WaveKey{(double)gv, gp, gg});
+          gv = (T)bk.v;
+          gp = bk.idx;
+          gg = bk.pay;
         }
         if (!s_abort) {
           const uint64_t* r = ws.rec + ((size_t)slot * GMAX + gg) * FREC;
