@@ -118,14 +118,16 @@ def test_dgetrf_singular_and_special(zh, lu_panel):
 
 @pytest.mark.parametrize("order", [ROW, COL])
 @pytest.mark.parametrize("trans", [b"N", b"T"])
-def test_dgetrs_and_dgesv(zh, lu_panel, order, trans):
-    n, nrhs = 333, 3
+@pytest.mark.parametrize("n,nrhs,ldb_extra", [(333, 3, 0), (333, 1, 0), (1100, 1, 2)])
+def test_dgetrs_and_dgesv(zh, lu_panel, order, trans, n, nrhs, ldb_extra):
+    """nrhs = 1 takes the one-vector path (interchanges in LDS chunks, one
+    persistent launch per triangle), also with a padded leading dimension."""
     a = O.fill_uniform(order, n, n, 77)
     A = dense(order, n, n, a, n)
     X = np.random.default_rng(1).uniform(-1, 1, (n, nrhs))
     Aop = A if trans == b"N" else A.T
     B = Aop @ X
-    ldb = n if order == COL else nrhs
+    ldb = (n if order == COL else nrhs) + ldb_extra
     ip = np.zeros(n, np.int32)
     lu = a.copy()
     assert zh.LAPACKE_dgetrf(order, n, n, P(lu), n, P(ip)) == 0
@@ -219,8 +221,8 @@ def test_dpotrf_not_positive_definite_later_block_column(zh, uplo):
 @pytest.mark.parametrize("order", [ROW, COL])
 @pytest.mark.parametrize("uplo", [b"L", b"U"])
 @pytest.mark.parametrize("n", [400, 1100])
-def test_dpotrs_dposv(zh, order, uplo, n):
-    nrhs = 16
+@pytest.mark.parametrize("nrhs", [16, 1])
+def test_dpotrs_dposv(zh, order, uplo, n, nrhs):
     A = _spd(n, 12)
     X = np.random.default_rng(5).uniform(-1, 1, (n, nrhs))
     B = A @ X

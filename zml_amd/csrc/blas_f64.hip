@@ -464,6 +464,274 @@ __global__ void lange_final_kernel(int mode, int64_t cnt, const double* __restri
 
 }  // namespace
 
+// ---- one right-hand side: persistent triangular solve and vector interchanges ----
+namespace {
+
+// x <- T^-1 x for one right-hand side, T n x n triangular with element (i, j)
+// at t[i * si + j * sj]. One persistent launch: T is cut into 64-row blocks,
+// workgroup g takes the block rows g, g + G, ... in solve order (top down for
+// lower, bottom up for upper). For its block row it accumulates
+//   s_i = sum over the blocks solved before it of T_ij x_j,
+// taking each block's x as soon as its owner has published it (several ready
+// blocks at a time), solves its diagonal block (one wave, x_j = (b_j - s_j) /
+// T_jj, or without the division for a unit diagonal) and publishes x of its
+// rows as tagged 64-bit words (tag << 32 | half of the double), read with
+// device-scope relaxed atomics: a reader knows a word is current from its tag.
+// The grid never exceeds the co-resident workgroups and a block row waits only
+// on block rows earlier in the solve order, so every wait ends; a wait that
+// exceeds its bound publishes NaN instead, which makes the result visibly wrong
+// rather than silently so. ROW: sj == 1, a row's elements are contiguous (four
+// threads share a row, 16 columns each); otherwise si == 1 and the lanes run
+// down a column.
+constexpr int T1B = 64;        // rows per block
+constexpr int T1BATCH = 8;     // published blocks taken per pass
+__device__ __forceinline__ void t1_put(uint64_t* p, uint32_t tag, double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  __hip_atomic_store(p, ((uint64_t)tag << 32) | (uint32_t)(b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, ((uint64_t)tag << 32) | (uint32_t)b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool t1_get(const uint64_t* p, uint32_t tag, double& v) {
+  const uint64_t hi = __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t lo = __hip_atomic_load(const_cast<uint64_t*>(p + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  v = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+  return (uint32_t)(hi >> 32) == tag && (uint32_t)(lo >> 32) == tag;
+}
+
+template <bool ROW>
+__global__ void __launch_bounds__(256)
+trsv1_kernel(int64_t n, const double* __restrict__ t, int64_t si, int64_t sj, int lower, int unit,
+             double* __restrict__ x, int64_t incx, uint64_t* __restrict__ pub, uint32_t tag, long spin_limit) {
+#pragma clang fp contract(off)
+  __shared__ double D[T1B][T1B + 1];
+  __shared__ double xs[T1BATCH][T1B];
+  __shared__ double part[4][T1B];
+  __shared__ int s_cnt, s_bad;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t nblk = (n + T1B - 1) / T1B;
+  const int64_t xoff = incx < 0 ? (1 - n) * incx : 0;
+  for (int64_t bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
+    const int64_t b = lower ? bi : nblk - 1 - bi;
+    const int64_t r0 = b * T1B;
+    const int rows = (int)(n - r0 < T1B ? n - r0 : T1B);
+    // the diagonal block (its triangle), read while the contributions come in
+    for (int e = tid; e < T1B * T1B; e += 256) {
+      const int i = e / T1B, j = e % T1B;
+      const bool in = i < rows && j < rows && (lower ? j <= i : j >= i);
+      D[i][j] = in ? t[(r0 + i) * si + (r0 + j) * sj] : 0.0;
+    }
+    // ROW: thread -> row tid / 4, columns 16 (tid % 4) .. +15 of each block;
+    // otherwise: thread -> row tid % 64, columns 16 (tid / 64) .. +15
+    const int myrow = ROW ? tid >> 2 : lane, q = ROW ? tid & 3 : wv;
+    const int64_t rowc = r0 + myrow < n ? r0 + myrow : n - 1;   // rows past n read row n - 1, unused
+    double acc = 0.0;
+    if (tid == 0) s_bad = 0;
+    for (int64_t jj = 0; jj < bi;) {
+      // wave 0 takes the next published blocks (the first one waited for)
+      if (wv == 0) {
+        int cnt = 0;
+        long spins = 0;
+        while (cnt < T1BATCH && jj + cnt < bi) {
+          const int64_t jb = lower ? jj + cnt : nblk - 1 - (jj + cnt);
+          const int64_t c0 = jb * T1B;
+          double v = 0.0;
+          const bool ok = c0 + lane >= n || t1_get(pub + 2 * (c0 + lane), tag, v);
+          if (__all(ok)) {
+            xs[cnt][lane] = c0 + lane < n ? v : 0.0;
+            ++cnt;
+            spins = 0;
+            continue;
+          }
+          if (cnt > 0) break;                 // take what is there
+          if (++spins > spin_limit) {         // give up: this block row becomes NaN
+            if (lane == 0) s_bad = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (lane == 0) s_cnt = cnt;
+      }
+      __syncthreads();
+      const int cnt = s_cnt;
+      if (s_bad) break;
+      for (int c = 0; c < cnt; ++c) {
+        const int64_t jb = lower ? jj + c : nblk - 1 - (jj + c);
+        const int64_t c0 = jb * T1B + 16 * q;
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const int64_t col = c0 + k < n ? c0 + k : n - 1;
+          v[k] = t[rowc * si + col * sj];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc += v[k] * xs[c][16 * q + k];
+      }
+      jj += cnt;
+      __syncthreads();   // xs is refilled next pass
+    }
+    // the four partial sums of each row
+    if (ROW) {
+      acc += __shfl_xor(acc, 1);
+      acc += __shfl_xor(acc, 2);
+      if (q == 0) part[0][myrow] = acc;
+    } else {
+      part[q][myrow] = acc;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      const bool bad = s_bad != 0;
+      double ti = 0.0;
+      if (lane < rows) {
+        const double s = ROW ? part[0][lane] : part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+        ti = x[xoff + (r0 + lane) * incx] - s;
+      }
+      // substitution inside the block, one column at a time
+      for (int k = 0; k < rows; ++k) {
+        const int kk = lower ? k : rows - 1 - k;
+        const uint64_t tb = (uint64_t)__double_as_longlong(ti);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(tb >> 32), kk);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)tb, kk);
+        double xk = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+        if (!unit) xk = xk / D[kk][kk];
+        if (lane == kk) ti = xk;
+        const bool below = lower ? lane > kk : lane < kk;
+        if (below && lane < rows) ti = ti - D[lane][kk] * xk;
+      }
+      if (bad) ti = __builtin_nan("");
+      if (lane < rows) {
+        x[xoff + (r0 + lane) * incx] = ti;
+        t1_put(pub + 2 * (r0 + lane), tag, ti);
+      }
+    }
+    __syncthreads();   // D and part are reused by the next block row
+  }
+}
+
+// The interchanges i <-> ipiv[ix] - 1 (k1..k2 in order, forward or backward)
+// applied to one vector with element stride ldx, by one workgroup: 256
+// interchanges at a time, the elements they touch gathered into LDS (an
+// open-addressing table gives each distinct row a slot), swapped there in order
+// by one lane, and scattered back.
+constexpr int LV_CH = 256, LV_HT = 2048;
+__global__ void __launch_bounds__(256)
+laswp_vec_kernel(double* __restrict__ xv, int64_t ldx, int k1, int k2, const int32_t* __restrict__ ipiv,
+                 int inc, int ix0, int incx) {
+  __shared__ int hkey[LV_HT], hslot[LV_HT];
+  __shared__ int rowof[2 * LV_CH], sa[LV_CH], sb[LV_CH];
+  __shared__ double val[2 * LV_CH];
+  __shared__ int s_n;
+  const int tid = threadIdx.x;
+  const int nsteps = inc > 0 ? k2 - k1 + 1 : k1 - k2 + 1;   // (k1..k2 stepping by inc)
+  for (int c0 = 0; c0 < nsteps; c0 += LV_CH) {
+    const int cn = nsteps - c0 < LV_CH ? nsteps - c0 : LV_CH;
+    for (int h = tid; h < LV_HT; h += 256) hkey[h] = -1;
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+    int ia = -1, ib = -1;
+    if (tid < cn) {
+      const int st = c0 + tid;
+      ia = inc > 0 ? k1 + st : k1 - st;
+      ib = ipiv[ix0 + st * incx] - 1;
+    }
+    // every distinct row of the chunk enters the table
+    for (int w = 0; w < 2; ++w) {
+      const int r = w == 0 ? ia : ib;
+      if (r >= 0) {
+        uint32_t h = ((uint32_t)r * 2654435761u) & (LV_HT - 1);
+        for (;;) {
+          const int prev = atomicCAS(&hkey[h], -1, r);
+          if (prev == -1 || prev == r) break;
+          h = (h + 1) & (LV_HT - 1);
+        }
+      }
+    }
+    __syncthreads();
+    for (int h = tid; h < LV_HT; h += 256) {
+      const int key = hkey[h];
+      if (key >= 0) {
+        const int sl = atomicAdd(&s_n, 1);
+        hslot[h] = sl;
+        rowof[sl] = key;
+        val[sl] = xv[(int64_t)key * ldx];
+      }
+    }
+    __syncthreads();
+    if (tid < cn) {
+      uint32_t h = ((uint32_t)ia * 2654435761u) & (LV_HT - 1);
+      while (hkey[h] != ia) h = (h + 1) & (LV_HT - 1);
+      sa[tid] = hslot[h];
+      h = ((uint32_t)ib * 2654435761u) & (LV_HT - 1);
+      while (hkey[h] != ib) h = (h + 1) & (LV_HT - 1);
+      sb[tid] = hslot[h];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int st = 0; st < cn; ++st) {
+        const int a = sa[st], bsl = sb[st];
+        if (a != bsl) {
+          const double tmp = val[a];
+          val[a] = val[bsl];
+          val[bsl] = tmp;
+        }
+      }
+    }
+    __syncthreads();
+    const int nn = s_n;
+    for (int sl = tid; sl < nn; sl += 256) xv[(int64_t)rowof[sl] * ldx] = val[sl];
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+struct Trsv1State {
+  int max_resident = 0;
+  uint32_t next_tag = 0;
+  void* ws = nullptr;
+  size_t bytes = 0;
+};
+static thread_local Trsv1State tl_trsv1[16];
+
+void trsv1_f64(int64_t n, const double* t, int64_t si, int64_t sj, bool lower, bool unit, double* x,
+               int64_t incx, hipStream_t s) {
+  if (n <= 0) return;
+  int dev = 0;
+  ZH_CHECK(hipGetDevice(&dev));
+  Trsv1State& st = tl_trsv1[dev & 15];
+  const int64_t nblk = (n + T1B - 1) / T1B;
+  const size_t bytes = sizeof(uint64_t) * 2 * (size_t)nblk * T1B;
+  void* ws = workspace(bytes, 25);
+  if (ws != st.ws || bytes > st.bytes || st.next_tag > 0x7fff0000u) {   // fresh words: every tag zero
+    ZH_CHECK(hipMemsetAsync(ws, 0, bytes, s));
+    st.ws = ws;
+    st.bytes = bytes;
+    st.next_tag = 0;
+  }
+  const bool row = sj == 1;
+  if (st.max_resident == 0) {
+    int a = 0, b = 0;
+    ZH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, trsv1_kernel<true>, 256, 0));
+    ZH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, trsv1_kernel<false>, 256, 0));
+    st.max_resident = std::max(1, std::min(a, b)) * device_cu_count();
+  }
+  const int grid = (int)std::min<int64_t>(nblk, std::min(st.max_resident, 1024));
+  const uint32_t tag = ++st.next_tag;
+  const long limit = 1L << 22;
+  if (row)
+    hipLaunchKernelGGL(trsv1_kernel<true>, dim3(grid), dim3(256), 0, s, n, t, si, sj, lower ? 1 : 0,
+                       unit ? 1 : 0, x, incx, static_cast<uint64_t*>(ws), tag, limit);
+  else
+    hipLaunchKernelGGL(trsv1_kernel<false>, dim3(grid), dim3(256), 0, s, n, t, si, sj, lower ? 1 : 0,
+                       unit ? 1 : 0, x, incx, static_cast<uint64_t*>(ws), tag, limit);
+  ZH_LAUNCH_CHECK();
+}
+
+void laswp_vec(double* x, int64_t ldx, int k1, int k2, const int32_t* ipiv, int inc, hipStream_t s) {
+  // the interchange of row k uses ipiv[k], in both orders
+  hipLaunchKernelGGL(laswp_vec_kernel, dim3(1), dim3(256), 0, s, x, ldx, k1, k2, ipiv, inc, k1, inc);
+  ZH_LAUNCH_CHECK();
+}
+
 void trsv_f64_cm(int uplo, int trans, int diag, int64_t n, const double* a, int64_t lda, double* x,
                  int64_t incx, hipStream_t s) {
   if (n <= 0) return;

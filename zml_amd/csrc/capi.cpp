@@ -284,6 +284,23 @@ int dgetrs_dev(int layout, char trans_c, int n, int nrhs, const double* a, int l
   if (lda < std::max(1, n)) return -6;
   if (ldb < std::max(1, layout == kColMajor ? n : nrhs)) return -9;
   if (n == 0 || nrhs == 0) return 0;
+  if (nrhs == 1) {
+    // one right-hand side: the vector interchanges and one persistent launch
+    // per triangle (element (i, j) of the factors at a[i si + j sj])
+    const int64_t si = layout == kRowMajor ? lda : 1, sj = layout == kRowMajor ? 1 : lda;
+    const int64_t inc = layout == kRowMajor ? ldb : 1;
+    const int32_t* ip = reinterpret_cast<const int32_t*>(ipiv);
+    if (is_notrans(trans)) {
+      laswp_vec(b, inc, 0, n - 1, ip, 1, s);
+      trsv1_f64(n, a, si, sj, true, true, b, inc, s);     // L, unit
+      trsv1_f64(n, a, si, sj, false, false, b, inc, s);   // U
+    } else {
+      trsv1_f64(n, a, sj, si, true, false, b, inc, s);    // U^T
+      trsv1_f64(n, a, sj, si, false, true, b, inc, s);    // L^T, unit
+      laswp_vec(b, inc, n - 1, 0, ip, -1, s);
+    }
+    return 0;
+  }
   auto trsm = [&](int side, int uplo, int t, int diag) {
     return dtrsm_dev(layout, side, uplo, t, diag, n, nrhs, 1.0, a, lda, b, ldb, s);
   };
@@ -334,6 +351,20 @@ int dpotrs_dev(int layout, char uplo_c, int n, int nrhs, const double* a, int ld
   if (lda < std::max(1, n)) return -6;
   if (ldb < std::max(1, layout == kColMajor ? n : nrhs)) return -8;
   if (n == 0 || nrhs == 0) return 0;
+  if (nrhs == 1) {
+    // one right-hand side: one persistent launch per triangle; the stored
+    // triangle T has element (i, j) at a[i si + j sj]
+    const int64_t si = layout == kRowMajor ? lda : 1, sj = layout == kRowMajor ? 1 : lda;
+    const int64_t inc = layout == kRowMajor ? ldb : 1;
+    if (uplo == kLower) {
+      trsv1_f64(n, a, si, sj, true, false, b, inc, s);    // L
+      trsv1_f64(n, a, sj, si, false, false, b, inc, s);   // L^T
+    } else {
+      trsv1_f64(n, a, sj, si, true, false, b, inc, s);    // U^T
+      trsv1_f64(n, a, si, sj, false, false, b, inc, s);   // U
+    }
+    return 0;
+  }
   auto trsm = [&](int u, int t) {
     return dtrsm_dev(layout, kLeft, u, t, kNonUnit, n, nrhs, 1.0, a, lda, b, ldb, s);
   };

@@ -186,6 +186,14 @@ void gemv_f64_cm(int trans, int64_t m, int64_t n, double alpha, const double* a,
 constexpr size_t kGemvScratchDoubles = size_t(1) << 20;   // 8 MiB per device
 int gemv_f64_cm_noalloc(int trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
                         const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s);
+// x <- T^-1 x for one right-hand side, T n x n triangular with element (i, j)
+// at t[i * si + j * sj] (si or sj must be 1), x strided by incx: one
+// persistent launch (blas_f64.hip). A bounded wait that gives up leaves NaN.
+void trsv1_f64(int64_t n, const double* t, int64_t si, int64_t sj, bool lower, bool unit, double* x,
+               int64_t incx, hipStream_t s);
+// The interchanges k1..k2 (inc = +1) or k1 down to k2 (inc = -1) of ipiv
+// (1-based, ipiv[k] for row k) applied to one vector of element stride ldx.
+void laswp_vec(double* x, int64_t ldx, int k1, int k2, const int32_t* ipiv, int inc, hipStream_t s);
 // x <- op(A)^-1 x, A n x n triangular (column-major view), x strided by incx.
 void trsv_f64_cm(int uplo, int trans, int diag, int64_t n, const double* a, int64_t lda, double* x,
                  int64_t incx, hipStream_t s);
