@@ -7,6 +7,14 @@
  * Real types only (f64 as zr_*, f32 as zr_s*, both from zml_oracle_real.inc):
  * conj_no_trans behaves as no_trans and conj_trans as trans, as zml's noconj
  * branches do for real types.
+ *
+ * Pins (tests/test_oracle.py): zml's own test vectors for gemm, trsm, syrk,
+ * gemv, trsv, iamax, scal and swap; the unblocked closed forms of getrf and of
+ * column-major lower potrf (bitwise, f64 and f32); hand-solved systems, IEEE
+ * constants for lamch, numpy for lange, a planted mistake that must fail.
+ * Parity unpinned: the bits of row-major and upper potrf / potrf2, which
+ * follow zml's blocked dot-form order with no reference-held vector or closed
+ * form to check them against (they are checked to 1e-12 only).
  */
 #include "zml_oracle.h"
 
