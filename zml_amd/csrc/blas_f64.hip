@@ -473,8 +473,8 @@ namespace {
 // lower, bottom up for upper). For its block row it accumulates
 //   s_i = sum over the blocks solved before it of T_ij x_j,
 // taking each block's x as soon as its owner has published it (several ready
-// blocks at a time), solves its diagonal block (one wave, x_j = (b_j - s_j) /
-// T_jj, or without the division for a unit diagonal) and publishes x of its
+// blocks at a time), solves its diagonal block (one wave, x_j = (b_j - s_j) *
+// fl(1 / T_jj), or without the scaling for a unit diagonal) and publishes x of its
 // rows as tagged 64-bit words (tag << 32 | half of the double), read with
 // device-scope relaxed atomics: a reader knows a word is current from its tag.
 // The grid never exceeds the co-resident workgroups and a block row waits only
@@ -525,17 +525,35 @@ trsv1_kernel(int64_t n, const double* __restrict__ t, int64_t si, int64_t sj, in
     const int64_t rowc = r0 + myrow < n ? r0 + myrow : n - 1;   // rows past n read row n - 1, unused
     double acc = 0.0;
     if (tid == 0) s_bad = 0;
+    auto load16 = [&](int64_t jblk, double (&v)[16]) {
+      const int64_t c0 = jblk * T1B + 16 * q;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int64_t col = c0 + k < n ? c0 + k : n - 1;
+        v[k] = t[rowc * si + col * sj];
+      }
+    };
     for (int64_t jj = 0; jj < bi;) {
+      // the factor's values of the next block do not depend on its x: they
+      // are requested before waiting for it
+      double vp[16];
+      load16(lower ? jj : nblk - 1 - jj, vp);
       // wave 0 takes the next published blocks (the first one waited for)
       if (wv == 0) {
+        // a block is looked at through one word (its last row's, written
+        // last) until that is current, then read whole; the many waiting
+        // workgroups then poll one word each instead of a block's 128
         int cnt = 0;
         long spins = 0;
         while (cnt < T1BATCH && jj + cnt < bi) {
           const int64_t jb = lower ? jj + cnt : nblk - 1 - (jj + cnt);
           const int64_t c0 = jb * T1B;
+          const int64_t last = (c0 + T1B < n ? c0 + T1B : n) - 1;
+          const uint64_t w = __hip_atomic_load(pub + 2 * last + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          bool ok = (uint32_t)(w >> 32) == tag;
           double v = 0.0;
-          const bool ok = c0 + lane >= n || t1_get(pub + 2 * (c0 + lane), tag, v);
-          if (__all(ok)) {
+          if (ok) ok = __all(c0 + lane >= n || t1_get(pub + 2 * (c0 + lane), tag, v));
+          if (ok) {
             xs[cnt][lane] = c0 + lane < n ? v : 0.0;
             ++cnt;
             spins = 0;
@@ -546,22 +564,18 @@ trsv1_kernel(int64_t n, const double* __restrict__ t, int64_t si, int64_t sj, in
             if (lane == 0) s_bad = 1;
             break;
           }
-          __builtin_amdgcn_s_sleep(1);
+          __builtin_amdgcn_s_sleep(4);
         }
         if (lane == 0) s_cnt = cnt;
       }
       __syncthreads();
       const int cnt = s_cnt;
       if (s_bad) break;
-      for (int c = 0; c < cnt; ++c) {
-        const int64_t jb = lower ? jj + c : nblk - 1 - (jj + c);
-        const int64_t c0 = jb * T1B + 16 * q;
-        double v[16];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          const int64_t col = c0 + k < n ? c0 + k : n - 1;
-          v[k] = t[rowc * si + col * sj];
-        }
+      for (int k = 0; k < 16; ++k) acc += vp[k] * xs[0][16 * q + k];
+      for (int c = 1; c < cnt; ++c) {
+        double v[16];
+        load16(lower ? jj + c : nblk - 1 - (jj + c), v);
 #pragma unroll
         for (int k = 0; k < 16; ++k) acc += v[k] * xs[c][16 * q + k];
       }
@@ -584,17 +598,30 @@ trsv1_kernel(int64_t n, const double* __restrict__ t, int64_t si, int64_t sj, in
         const double s = ROW ? part[0][lane] : part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
         ti = x[xoff + (r0 + lane) * incx] - s;
       }
-      // substitution inside the block, one column at a time
+      // substitution inside the block, one column at a time. The diagonal's
+      // reciprocals are formed in parallel beforehand (x_k = t_k fl(1/d_k):
+      // a division per step would sit on the serial chain), and each lane's
+      // next element of the block is read from LDS one step ahead, so only the
+      // broadcast and two operations per step are serial.
+      const double rinv = (!unit && lane < rows) ? 1.0 / D[lane][lane] : 1.0;
+      int kk = lower ? 0 : rows - 1;
+      double dcur = D[lane][kk];
       for (int k = 0; k < rows; ++k) {
-        const int kk = lower ? k : rows - 1 - k;
-        const uint64_t tb = (uint64_t)__double_as_longlong(ti);
+        const int kn = lower ? kk + 1 : kk - 1;
+        const int kc = k + 1 < rows ? kn : kk;   // (an index inside the block either way)
+        const double dnext = D[lane][kc];
+        const uint64_t tb = (uint64_t)__double_as_longlong(ti), rb = (uint64_t)__double_as_longlong(rinv);
         const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(tb >> 32), kk);
         const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)tb, kk);
-        double xk = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-        if (!unit) xk = xk / D[kk][kk];
+        const uint32_t rhi = (uint32_t)__builtin_amdgcn_readlane((int)(rb >> 32), kk);
+        const uint32_t rlo = (uint32_t)__builtin_amdgcn_readlane((int)rb, kk);
+        const double xk = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo)) *
+                          __longlong_as_double((long long)(((uint64_t)rhi << 32) | rlo));
         if (lane == kk) ti = xk;
         const bool below = lower ? lane > kk : lane < kk;
-        if (below && lane < rows) ti = ti - D[lane][kk] * xk;
+        if (below && lane < rows) ti = ti - dcur * xk;
+        kk = kn;
+        dcur = dnext;
       }
       if (bad) ti = __builtin_nan("");
       if (lane < rows) {
