@@ -253,3 +253,36 @@ def test_lu_n16384_pivots_match_zml(zh, lu_panel):
     assert mism.size == 0, f"pivots differ from zml's first at column {mism[0]}"
     diag = np.diagonal(a.reshape(n, n).T)
     assert np.abs(diag - g["diag"]).max() <= 1e-9 * np.abs(g["diag"]).max()
+
+
+@pytest.mark.parametrize("order", [ROW, COL])
+@pytest.mark.parametrize("trans", [b"N", b"T"])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129])
+def test_one_vector_getrs_potrs_small_orders(zh, order, trans, n):
+    """The one-vector solves at orders around their 64-row block: partial and
+    single blocks, one row, against the oracle's getrs / potrs."""
+    a = O.fill_uniform(order, n, n, 90 + n)
+    A = dense(order, n, n, a, n)
+    x0 = np.random.default_rng(n).uniform(-1, 1, (n, 1))
+    B = (A if trans == b"N" else A.T) @ x0
+    ldb = n if order == COL else 1
+    ip = np.zeros(n, np.int32)
+    lu = a.copy()
+    assert zh.LAPACKE_dgetrf(order, n, n, P(lu), n, P(ip)) == 0
+    b = store(order, B, ldb)
+    assert zh.LAPACKE_dgetrs(order, trans, n, 1, P(lu), n, P(ip), P(b), ldb) == 0
+    lu_ref = a.copy()
+    ip_ref = np.zeros(n, np.int32)
+    O.getrf(order, n, n, lu_ref, n, ip_ref)
+    b_ref = store(order, B, ldb)
+    O.getrs(order, 111 if trans == b"N" else 112, n, 1, lu_ref, n, ip_ref, b_ref, ldb)
+    Xg, Xr = dense(order, n, 1, b, ldb), dense(order, n, 1, b_ref, ldb)
+    assert np.abs(Xg - Xr).max() <= 1e-10 * max(1.0, np.abs(Xr).max())
+    if trans == b"N":
+        S = _spd(n, 30 + n)
+        s_ = store(order, S, n)
+        assert zh.LAPACKE_dpotrf(order, b"L", n, P(s_), n) == 0
+        y = S @ x0
+        yb = store(order, y, ldb)
+        assert zh.LAPACKE_dpotrs(order, b"L", n, 1, P(s_), n, P(yb), ldb) == 0
+        assert np.abs(dense(order, n, 1, yb, ldb) - x0).max() < 1e-11

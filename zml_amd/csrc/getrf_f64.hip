@@ -365,11 +365,10 @@ static const int32_t* fused_block_perm(int64_t j0) {
 }
 
 // The share of a trailing update to apply beside the next panel (on the CUs
-// the panel leaves free): about the panel's duration over the update's. The
-// figures are one MI355X's: a 512-column FP64 panel takes about 5 ms whatever
-// its height (its columns are latency bound), the update runs at about
-// 44 TF/s on the 192 CUs the panel leaves; the rest of the update then runs on
-// all 256 CUs once the panel is done.
+// the panel leaves free): a fixed 5 ms of update on 192 CUs at 44 TF/s, the
+// rest then runs on all 256 CUs once the panel is done. (Measured on one
+// MI355X at n = 32768: 718 ms per LU; sizing it to the one-launch panel's
+// 7.4 ms plus 20% gave 729 ms.)
 static double rest_split_f64(int64_t rows, int64_t cols, int64_t kb, int64_t panel_rows) {
   (void)panel_rows;
   const double t_panel = 5.0e-3;
