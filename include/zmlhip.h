@@ -127,6 +127,12 @@ int zmlhip_set_panel_spin_limit(long polls);
  * MI355X); a negative value restores it, 0 makes every block column take the
  * one-launch panel. Process-wide. No counterpart in the reference. */
 int zmlhip_set_panel_one_launch_rows(long rows);
+/* FP64 GEMMs (zmlhip_dgemm and the GEMMs inside the FP64 factorizations)
+ * whose shapes allow it move their operand slabs from global memory to LDS
+ * with the load unit (on = 1, the default); on = 0 makes every FP64 GEMM stage
+ * them through registers. The two kernels give the same results bit for bit.
+ * Returns the previous setting. Process-wide. No counterpart in the reference. */
+int zmlhip_set_gemm_lds_dma(int on);
 /* LAPACKE_dgetrs (src/linalg/lapack.zig:1575; lapack/getrs.zig:13-142) */
 int zmlhip_dgetrs(int layout, char trans, int n, int nrhs, const double *a, int lda,
                   const int *ipiv, double *b, int ldb, void *stream);

@@ -540,6 +540,12 @@ int zmlhip_set_panel_one_launch_rows(long rows) {
   return 0;
 }
 
+int zmlhip_set_gemm_lds_dma(int on) {
+  const int was = gemm_dma_enabled() ? 1 : 0;
+  set_gemm_dma(on);
+  return was;
+}
+
 int zmlhip_dlaswp(int layout, int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx,
                   void* stream) {
   return fenced(as_stream(stream), [&] { return dlaswp_dev(layout, n, a, lda, k1, k2, ipiv, incx, as_stream(stream)); });

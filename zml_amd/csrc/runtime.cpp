@@ -199,6 +199,12 @@ void set_panel_one_launch_rows(long rows) {
   g_one_launch_rows.store(rows < 0 ? kOneLaunchRowsDefault : rows, std::memory_order_relaxed);
 }
 
+namespace {
+std::atomic<int> g_gemm_dma{1};
+}
+bool gemm_dma_enabled() { return g_gemm_dma.load(std::memory_order_relaxed) != 0; }
+void set_gemm_dma(int on) { g_gemm_dma.store(on ? 1 : 0, std::memory_order_relaxed); }
+
 int device_cu_count() {
   int d = 0;
   ZH_CHECK(hipGetDevice(&d));

@@ -125,6 +125,11 @@ void set_panel_spin_limit(long polls);
 // FP64 LU block columns with more rows than this take the one-launch panel.
 long panel_one_launch_rows();
 void set_panel_one_launch_rows(long rows);
+// FP64 GEMMs whose shapes allow it move their operand slabs global -> LDS with
+// the load unit (default); off, every FP64 GEMM stages them through registers.
+// Both kernels give the same results bit for bit.
+bool gemm_dma_enabled();
+void set_gemm_dma(int on);
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
