@@ -186,12 +186,6 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
     const int64_t jb0 = std::min(nb, mn);
     panel_end(0, jb0, 0, panel_begin(0, jb0));
   }
-  // a trailing update split in two: its second part runs on the caller's
-  // stream once the next panel is done (ev_split_u marks its end; the next
-  // step's work on the same columns waits for it)
-  const hipEvent_t ev_split_s = aux_event(8), ev_split_u = aux_event(9);
-  const bool can_split = s != caller && ops.rest_split && !ops.try_panel_finish;
-  bool split_pending = false;
   for (int64_t j0 = 0, blk = 0; j0 < mn; j0 += nb, ++blk) {
     const int64_t jb = std::min(nb, mn - j0);
     const T* invb = ops.inv_store ? ops.inv_store(j0) : inv + (blk & 1) * nb * nb;
@@ -218,26 +212,13 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
     }
     // the rest: interchanges left of the block and beyond the next one, then
     // update (queued before the next panel's outcome, which waits on the host)
-    // the previous step's second part reads that step's L21 (which the
-    // interchanges left of this block move) and writes columns of this rest
-    if (split_pending) ZH_CHECK(hipStreamWaitEvent(s, ev_split_u, 0));
-    split_pending = false;
     apply_perm<T>(j0, A, lda, pr, s);
     const int64_t c2 = c1 + jb2, nr = n - c2;
-    int64_t nr1 = nr;   // columns updated here, beside the next panel
     if (nr > 0) {
       apply_perm<T>(nr, A + c2, lda, pr, s);
       ops.apply_inv(jb, nr, invb, A + j0 * lda + c2, lda, tmp, s);
-      if (can_split && jb2 > 0 && rows > 0) {
-        const double f = ops.rest_split(rows, nr, jb, m - c1);
-        // whole 128-column tiles, and at least the block column after the next
-        // (the next step updates it first, without waiting for the second part)
-        int64_t c = (int64_t)(f * (double)nr + 127.0) / 128 * 128;
-        c = std::max<int64_t>(c, std::min<int64_t>(nb, nr));
-        nr1 = std::min<int64_t>(c, nr);
-      }
-      if (rows > 0 && nr1 > 0)
-        ops.gemm(rows, nr1, jb, A + c1 * lda + j0, lda, A + j0 * lda + c2, lda, A + c1 * lda + c2, lda, s);
+      if (rows > 0)
+        ops.gemm(rows, nr, jb, A + c1 * lda + j0, lda, A + j0 * lda + c2, lda, A + c1 * lda + c2, lda, s);
     }
     if (jb2 > 0) {
       if (!begun) {
@@ -245,17 +226,6 @@ void getrf_lookahead(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int
         attempted = panel_begin(c1, jb2);
       }
       panel_end(c1, jb2, blk + 1, attempted);
-    }
-    if (nr1 < nr && rows > 0) {
-      // the second part: after this step's interchanges and U12 (on s) and
-      // after the next panel (whose CUs it then also uses)
-      ZH_CHECK(hipEventRecord(ev_split_s, s));
-      ZH_CHECK(hipStreamWaitEvent(caller, ev_split_s, 0));
-      ZH_CHECK(hipStreamWaitEvent(caller, ev_panel, 0));
-      ops.gemm(rows, nr - nr1, jb, A + c1 * lda + j0, lda, A + j0 * lda + c2 + nr1, lda, A + c1 * lda + c2 + nr1,
-               lda, caller);
-      ZH_CHECK(hipEventRecord(ev_split_u, caller));
-      split_pending = true;
     }
   }
   if (s != caller) {
@@ -369,12 +339,6 @@ static const int32_t* fused_block_perm(int64_t j0) {
 // rest then runs on all 256 CUs once the panel is done. (Measured on one
 // MI355X at n = 32768: 718 ms per LU; sizing it to the one-launch panel's
 // 7.4 ms plus 20% gave 729 ms.)
-static double rest_split_f64(int64_t rows, int64_t cols, int64_t kb, int64_t panel_rows) {
-  (void)panel_rows;
-  const double t_panel = 5.0e-3;
-  const double t_update = 2.0 * (double)rows * (double)cols * (double)kb / 44.0e12;
-  return t_update > 0.0 ? std::min(1.0, t_panel / t_update) : 1.0;
-}
 
 // Plain partial-pivoting LU of a row-major matrix in T (FP64 for dgetrf, FP32
 // for sgetrf): the same driver, leaves and kernels, no low-precision update.
@@ -409,7 +373,6 @@ void getrf_rm(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info,
     ops.panel = fused_panel_f64;
     ops.block_perm = fused_block_perm;
     ops.panel_min_rows = kFusedMinRows;
-    ops.rest_split = rest_split_f64;
   }
   getrf_lookahead<T>(m, n, A, lda, ipiv, info, NB_F64, ops, s);
 }

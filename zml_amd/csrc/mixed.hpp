@@ -70,12 +70,6 @@ struct LuOps {
   void (*panel)(int64_t m, T* A, int64_t lda, int64_t j0, int64_t jb, int32_t* ipiv, int* info,
                 hipStream_t s) = nullptr;
   int64_t panel_min_rows = 0;
-  // optional (drivers without try_panel_finish, on a CU-sparing stream): the
-  // fraction of the trailing update's columns (rows x cols, inner kb) to apply
-  // beside the next panel on the sparing stream; the rest waits for that
-  // panel and then runs on the caller's stream, over every CU. A scheduling
-  // estimate only: the results do not depend on it.
-  double (*rest_split)(int64_t rows, int64_t cols, int64_t kb, int64_t panel_rows) = nullptr;
 };
 
 int dposv_mixed(int layout, int uplo, int64_t n, int64_t nrhs, const double* A, int64_t lda,
