@@ -165,6 +165,8 @@ def main():
     ap.add_argument("--precision", default="fp16", choices=sorted(PREC))
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-host-abi", action="store_true",
+                    help="plain workload: skip the end-to-end LAPACKE_dgesv timing through host pointers")
     ap.add_argument("--workload", default="lu", choices=["lu", "cholesky", "batched", "plain"])
     ap.add_argument("--systems", type=int, default=20000)
     ap.add_argument("--mu", type=float, default=1e-2)
@@ -365,6 +367,39 @@ def bench_plain(args):
     assert int(info.item()) == 0
     be_lu = be_of(A, x, A.abs().sum(1).max().item())
 
+    # The same solve through the host-pointer drop-in (LAPACKE_dgesv, row-major, pageable host
+    # buffers): each call allocates its device buffers, copies A and b in over PCIe, factors,
+    # solves and copies LU, ipiv and x back. Reported beside the device-resident figure, never
+    # as the value.
+    host_abi = None
+    if not args.no_host_abi:
+        import numpy as np
+        a_host = A.cpu().numpy()
+        b_host = b.cpu().numpy()
+        x_dev = x.cpu().numpy()
+        vp = lambda arr: arr.ctypes.data_as(ctypes.c_void_p)
+
+        def host_step():
+            a_c, b_c, ip = a_host.copy(), b_host.copy(), np.zeros(n, np.int32)
+            t0 = time.perf_counter()
+            rc = zh.LAPACKE_dgesv(ROW, n, 1, vp(a_c), n, vp(ip), vp(b_c), 1)
+            dt = time.perf_counter() - t0
+            if rc != 0:
+                raise RuntimeError(f"LAPACKE_dgesv returned {rc}: {_lib.last_error()}")
+            return dt, b_c
+
+        host_step()
+        runs = [host_step() for _ in range(max(1, min(args.steps, 3)))]
+        th = sorted(r[0] for r in runs)
+        t_host = th[len(th) // 2]
+        host_abi = {"entry": "LAPACKE_dgesv (host pointers, pageable memory)",
+                    "ms": round(t_host * 1e3, 2), "tflops": round((2.0 / 3.0) * n ** 3 / t_host / 1e12, 3),
+                    "pcie_bytes": int(8 * (2 * n * n + 2 * n) + 4 * n),
+                    "ms_all": [round(t * 1e3, 2) for t in th],
+                    "x_bitwise_equal_to_device_path": bool(np.array_equal(runs[-1][1].reshape(-1),
+                                                                          x_dev.reshape(-1)))}
+        del a_host
+
     # SPD: A = B B^T / n + I (B ~ N(0, 1)), column-major lower
     Bm = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
     S = Bm @ Bm.T
@@ -424,6 +459,7 @@ def bench_plain(args):
                   "frac_fp64_peak": round(f_gm / t_gm / 1e12 / PEAK["fp64"], 4)},
         "roofline": roofline_of("gemm_f64 trailing update inside dgetrf", ms.value, nl.value, fl.value, by.value,
                                 PEAK["fp64"], args.steps, None),
+        "host_abi": host_abi,
     }
     print(json.dumps(out))
 
