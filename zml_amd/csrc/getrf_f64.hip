@@ -261,6 +261,144 @@ static void invert_unit_lower(int64_t kb, const T* L11, int64_t ld, T* inv, hipS
   ZH_LAUNCH_CHECK();
   trsm_update<T>(kb, kb, L11, ld, inv, kb, s);
 }
+
+// ---- X = L^-1 of a row-major unit-lower kb x kb block (FP64), in 64 x 64
+// blocks: first every diagonal block X_ii = L_ii^-1 (one workgroup per block;
+// lane c of one wave forward-substitutes column c, x_rc = -sum_{k=c}^{r-1}
+// l_rk x_kc with k ascending), then, for the block distances d = 1, 2, ... in
+// turn, every block with i - j = d at once:
+//   X_ij = -X_ii (sum_{k=j}^{i-1} L_ik X_kj)
+// (one launch per distance, one workgroup per block, the 64 x 64 products on
+// FP64 MFMA). The strictly upper part of X is written as zeros. kb / 64
+// launches in all, against two per 32 columns for the blocked right-side
+// triangular solve of the identity (invert_unit_lower); the values are those
+// of another summation order.
+namespace invblk {
+constexpr int IB = 64, ILD = IB + 1;
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// 64 x 64 block of row-major rows [r0, r0 + 64) and columns [c0, c0 + 64) into
+// LDS (row stride ILD); rows from nr on and columns from nc on are zeros (their
+// loads are clamped into the block), all loads issued first
+__device__ __forceinline__ void stage(double* __restrict__ S, const double* __restrict__ G, int64_t ld, int r0,
+                                      int c0, int nr, int nc, int t) {
+  const int b = t & 63, bc = b < nc ? b : nc - 1;
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int a = (t >> 6) + 4 * q;
+    v[q] = G[(int64_t)(r0 + (a < nr ? a : nr - 1)) * ld + c0 + bc];
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int a = (t >> 6) + 4 * q;
+    S[a * ILD + b] = (a < nr && b < nc) ? v[q] : 0.0;
+  }
+}
+
+// acc += P Q for the 32 x 32 quadrant (wm, wn) of a 64 x 64 product, P and Q
+// 64 x 64 in LDS. Lane l supplies P[row l % 16][k l / 16] and Q[k l / 16][col
+// l % 16] of each 16 x 16 x 4 step and holds C[row 4 r + l / 16][col l % 16].
+__device__ __forceinline__ void mma64(const double* __restrict__ P, const double* __restrict__ Q, d4 (&acc)[2][2],
+                                      int wm, int wn, int lane) {
+  const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int ks = 0; ks < IB / 4; ++ks) {
+    const int k = 4 * ks + lk;
+    double a[2], b[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) a[x] = P[(wm * 32 + 16 * x + lr) * ILD + k];
+#pragma unroll
+    for (int y = 0; y < 2; ++y) b[y] = Q[k * ILD + wn * 32 + 16 * y + lr];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+  }
+}
+
+__global__ void __launch_bounds__(256) diag_kernel(int kb, const double* __restrict__ L, int64_t ld,
+                                                   double* __restrict__ X, int64_t ldx) {
+  __shared__ double Ls[IB * ILD], Xs[IB * ILD];
+  const int i = blockIdx.x, r0 = i * IB, nr = min(IB, kb - r0);
+  const int t = threadIdx.x;
+  stage(Ls, L, ld, r0, r0, nr, nr, t);
+  __syncthreads();
+  if (t < IB) {
+    const int c = t;
+    for (int r = 0; r < IB; ++r) {
+      double v = r == c ? 1.0 : 0.0;
+      if (c < r && r < nr) {
+        double acc = 0.0;
+        for (int k = c; k < r; ++k) acc = fma(Ls[r * ILD + k], Xs[k * ILD + c], acc);
+        v = -acc;
+      }
+      Xs[r * ILD + c] = v;
+    }
+  }
+  __syncthreads();
+  // block row i of X: the diagonal block, zeros right of it
+  for (int a = t >> 6; a < nr; a += 4)
+    for (int b = r0 + (t & 63); b < kb; b += 64)
+      X[(int64_t)(r0 + a) * ldx + b] = b < r0 + nr ? Xs[a * ILD + (b - r0)] : 0.0;
+}
+
+__global__ void __launch_bounds__(256) dist_kernel(int kb, int d, const double* __restrict__ L, int64_t ld,
+                                                   double* __restrict__ X, int64_t ldx) {
+  __shared__ double Ps[IB * ILD], Qs[IB * ILD];
+  const int i = d + blockIdx.x, j = i - d, r0 = i * IB, c0 = j * IB, nr = min(IB, kb - r0);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w & 1, wn = w >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  d4 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+  // S = sum_{k=j}^{i-1} L_ik X_kj (blocks k < i are whole)
+  for (int kk = j; kk < i; ++kk) {
+    __syncthreads();
+    stage(Ps, L, ld, r0, kk * IB, nr, IB, t);
+    stage(Qs, X, ldx, kk * IB, c0, IB, IB, t);
+    __syncthreads();
+    mma64(Ps, Qs, acc, wm, wn, lane);
+  }
+  __syncthreads();
+  // S to Qs, X_ii (rows and columns from nr on zero) to Ps
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        Qs[(wm * 32 + 16 * x + 4 * r + lk) * ILD + wn * 32 + 16 * y + lr] = acc[x][y][r];
+  stage(Ps, X, ldx, r0, r0, nr, nr, t);
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+  mma64(Ps, Qs, acc, wm, wn, lane);
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = wm * 32 + 16 * x + 4 * r + lk;
+        if (a < nr) X[(int64_t)(r0 + a) * ldx + c0 + wn * 32 + 16 * y + lr] = -acc[x][y][r];
+      }
+}
+}  // namespace invblk
+
+static void invert_unit_lower_blocked_f64(int64_t kb, const double* L11, int64_t ld, double* inv, hipStream_t s) {
+  const int nbk = (int)((kb + invblk::IB - 1) / invblk::IB);
+  hipLaunchKernelGGL(invblk::diag_kernel, dim3(nbk), dim3(256), 0, s, (int)kb, L11, ld, inv, kb);
+  ZH_LAUNCH_CHECK();
+  for (int d = 1; d < nbk; ++d) {
+    hipLaunchKernelGGL(invblk::dist_kernel, dim3(nbk - d), dim3(256), 0, s, (int)kb, d, L11, ld, inv, kb);
+    ZH_LAUNCH_CHECK();
+  }
+}
 template <typename T>
 static void apply_inv(int64_t kb, int64_t cols, const T* inv, T* B, int64_t ldb, T* tmp, hipStream_t s) {
   ZH_CHECK(hipMemcpy2DAsync(tmp, sizeof(T) * cols, B, sizeof(T) * ldb, sizeof(T) * cols, kb,
@@ -370,6 +508,7 @@ void getrf_rm(int64_t m, int64_t n, T* A, int64_t lda, int32_t* ipiv, int* info,
   LuOps<T> ops{gemm_update<T>, trsm_update<T>, invert_unit_lower<T>, apply_inv<T>, nullptr, nullptr,
                nullptr, spare, nullptr};
   if constexpr (sizeof(T) == 8) {
+    ops.invert = invert_unit_lower_blocked_f64;
     ops.panel = fused_panel_f64;
     ops.block_perm = fused_block_perm;
     ops.panel_min_rows = kFusedMinRows;
