@@ -367,6 +367,9 @@ constexpr int BKF = 16;
 constexpr int LDAF = BKF + 1;        // A rows in LDS: [i][k], padded
 constexpr int LDBF = NTL;            // B rows in LDS: [k][j]
 
+// VEC: A, B 16-byte aligned with lda, ldb (and the K slice) multiples of 4, so
+// whole 8-float runs inside the matrices load as two float4 each.
+template <bool VEC>
 __global__ void __launch_bounds__(NTH, 2)
 mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __restrict__ A,
                  int64_t lda, const float* __restrict__ B, int64_t ldb, float beta,
@@ -395,22 +398,53 @@ mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __re
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
-  // A tile 128 x 16: thread -> row t / 2 (8 floats); B tile 16 x 128: row t / 16 (8 floats)
+  // A tile 128 x 16: thread -> row t / 2 (8 floats); B tile 16 x 128: row t / 16 (8 floats).
+  // Every load stays in bounds: rows of A beyond m and columns of B beyond n
+  // read the last valid one (their products land only in C entries that are not
+  // stored); k beyond K reads the last valid k and is zeroed after the load.
   float ra[8], rb[8];
   auto load = [&](int64_t k0) {
-    const int r = t >> 1, kc = (t & 1) * 8;
-    const int64_t ia = row0 + r;
+    if (K <= 0) {   // nothing to load (the product is zero)
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int64_t kk = k0 + kc + e;
-      ra[e] = (ia < m && kk < K) ? A[ia * lda + kk] : 0.0f;
+      for (int e = 0; e < 8; ++e) ra[e] = rb[e] = 0.0f;
+      return;
+    }
+    const int r = t >> 1, kc = (t & 1) * 8;
+    const int64_t ia = row0 + r < m ? row0 + r : m - 1;
+    const float* pa = A + ia * lda;
+    if (VEC && k0 + kc + 8 <= K) {
+      const float4 v0 = *reinterpret_cast<const float4*>(pa + k0 + kc);
+      const float4 v1 = *reinterpret_cast<const float4*>(pa + k0 + kc + 4);
+      ra[0] = v0.x; ra[1] = v0.y; ra[2] = v0.z; ra[3] = v0.w;
+      ra[4] = v1.x; ra[5] = v1.y; ra[6] = v1.z; ra[7] = v1.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int64_t kk = k0 + kc + e;
+        ra[e] = pa[kk < K ? kk : K - 1];
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (k0 + kc + e >= K) ra[e] = 0.0f;
     }
     const int kr = t >> 4, jc = (t & 15) * 8;
     const int64_t kk = k0 + kr;
+    const float* pb = B + (kk < K ? kk : K - 1) * ldb;
+    if (VEC && col0 + jc + 8 <= n) {
+      const float4 v0 = *reinterpret_cast<const float4*>(pb + col0 + jc);
+      const float4 v1 = *reinterpret_cast<const float4*>(pb + col0 + jc + 4);
+      rb[0] = v0.x; rb[1] = v0.y; rb[2] = v0.z; rb[3] = v0.w;
+      rb[4] = v1.x; rb[5] = v1.y; rb[6] = v1.z; rb[7] = v1.w;
+    } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int64_t j = col0 + jc + e;
-      rb[e] = (kk < K && j < n) ? B[kk * ldb + j] : 0.0f;
+      for (int e = 0; e < 8; ++e) {
+        const int64_t j = col0 + jc + e;
+        rb[e] = pb[j < n ? j : n - 1];
+      }
+    }
+    if (kk >= K) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) rb[e] = 0.0f;
     }
   };
   auto store = [&](int buf) {
@@ -457,6 +491,13 @@ mp_gemm_f_kernel(int64_t m, int64_t n, int64_t K, float alpha, const float* __re
     __syncthreads();
   }
   epilogue(acc, cv, m, n, alpha, beta, rs, cs, C, ldc, row0 + wm * 64, col0 + wn * 64, lane);
+}
+
+// mp_gemm_f_kernel's float4 loads apply: 16-byte aligned operands, leading
+// dimensions (and the K offset of a split-K slice) multiples of 4 floats
+bool f32_vec_ok(const float* A, int64_t lda, const float* B, int64_t ldb, int64_t kslice) {
+  return lda % 4 == 0 && ldb % 4 == 0 && kslice % 4 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
+         (reinterpret_cast<uintptr_t>(B) & 15) == 0;
 }
 
 // the streamed variant applies: whole 32-wide k-slabs, 16-byte aligned rows
@@ -506,8 +547,12 @@ void mp_gemm_f32(int64_t m, int64_t n, int64_t K, float alpha, const float* A, i
   const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
   const int64_t tri_tiles = lower ? tri_tile_count(kLower, ntm, ntn) : 0;
   const int64_t tiles = tri_tiles > 0 ? tri_tiles : ntm * ntn;
-  hipLaunchKernelGGL(mp_gemm_f_kernel, dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha, A, lda,
-                     B, ldb, beta, rs, cs, C, ldc, lower, tri_tiles, (int64_t)0, (float*)nullptr);
+  if (f32_vec_ok(A, lda, B, ldb, K))
+    hipLaunchKernelGGL(mp_gemm_f_kernel<true>, dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha, A, lda,
+                       B, ldb, beta, rs, cs, C, ldc, lower, tri_tiles, (int64_t)0, (float*)nullptr);
+  else
+    hipLaunchKernelGGL(mp_gemm_f_kernel<false>, dim3((unsigned)tiles), dim3(NTH), 0, s, m, n, K, alpha, A, lda,
+                       B, ldb, beta, rs, cs, C, ldc, lower, tri_tiles, (int64_t)0, (float*)nullptr);
   ZH_LAUNCH_CHECK();
 }
 
@@ -536,9 +581,14 @@ void mp_gemm_f32_splitk(int64_t m, int64_t n, int64_t K, float alpha, const floa
   const int64_t slices = std::max<int64_t>(1, std::min<int64_t>(8, K / 64));
   const int64_t kslice = (K + slices - 1) / slices;
   const int64_t ntm = (m + MT - 1) / MT, ntn = (n + NTL - 1) / NTL;
-  hipLaunchKernelGGL(mp_gemm_f_kernel, dim3((unsigned)(ntm * ntn), (unsigned)slices), dim3(NTH), 0, s, m, n, K,
-                     1.0f, A, lda, B, ldb, 0.0f, (const float*)nullptr, (const float*)nullptr, C, ldc, 0,
-                     (int64_t)0, kslice, part);
+  if (f32_vec_ok(A, lda, B, ldb, kslice))
+    hipLaunchKernelGGL(mp_gemm_f_kernel<true>, dim3((unsigned)(ntm * ntn), (unsigned)slices), dim3(NTH), 0, s, m, n,
+                       K, 1.0f, A, lda, B, ldb, 0.0f, (const float*)nullptr, (const float*)nullptr, C, ldc, 0,
+                       (int64_t)0, kslice, part);
+  else
+    hipLaunchKernelGGL(mp_gemm_f_kernel<false>, dim3((unsigned)(ntm * ntn), (unsigned)slices), dim3(NTH), 0, s, m,
+                       n, K, 1.0f, A, lda, B, ldb, 0.0f, (const float*)nullptr, (const float*)nullptr, C, ldc, 0,
+                       (int64_t)0, kslice, part);
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256)), dim3(256), 0, s, m, n,
                      (int)slices, (const float*)part, alpha, beta, C, ldc);
   ZH_LAUNCH_CHECK();
