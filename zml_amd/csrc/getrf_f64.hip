@@ -264,8 +264,8 @@ static void invert_unit_lower(int64_t kb, const T* L11, int64_t ld, T* inv, hipS
 
 // ---- X = L^-1 of a row-major unit-lower kb x kb block (FP64), in 64 x 64
 // blocks: first every diagonal block X_ii = L_ii^-1 (one workgroup per block;
-// lane c of one wave forward-substitutes column c, x_rc = -sum_{k=c}^{r-1}
-// l_rk x_kc with k ascending), then, for the block distances d = 1, 2, ... in
+// lane c of one wave forms column c, x_rc = -sum_{k=c}^{r-1} l_rk x_kc with k
+// ascending, in registers), then, for the block distances d = 1, 2, ... in
 // turn, every block with i - j = d at once:
 //   X_ij = -X_ii (sum_{k=j}^{i-1} L_ik X_kj)
 // (one launch per distance, one workgroup per block, the 64 x 64 products on
@@ -325,16 +325,19 @@ __global__ void __launch_bounds__(256) diag_kernel(int kb, const double* __restr
   stage(Ls, L, ld, r0, r0, nr, nr, t);
   __syncthreads();
   if (t < IB) {
+    // column c of L_ii^-1 in registers, right-looking: x_r -= l_rk x_k for
+    // k ascending (terms with k < c are exact zeros); every lane reads the same
+    // l_rk (an LDS broadcast), the loops fully unrolled
     const int c = t;
-    for (int r = 0; r < IB; ++r) {
-      double v = r == c ? 1.0 : 0.0;
-      if (c < r && r < nr) {
-        double acc = 0.0;
-        for (int k = c; k < r; ++k) acc = fma(Ls[r * ILD + k], Xs[k * ILD + c], acc);
-        v = -acc;
-      }
-      Xs[r * ILD + c] = v;
-    }
+    double x[IB];
+#pragma unroll
+    for (int r = 0; r < IB; ++r) x[r] = r == c ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < IB; ++k)
+#pragma unroll
+      for (int r = k + 1; r < IB; ++r) x[r] = fma(-Ls[r * ILD + k], x[k], x[r]);
+#pragma unroll
+    for (int r = 0; r < IB; ++r) Xs[r * ILD + c] = x[r];
   }
   __syncthreads();
   // block row i of X: the diagonal block, zeros right of it
