@@ -133,6 +133,12 @@ int zmlhip_set_panel_one_launch_rows(long rows);
  * them through registers. The two kernels give the same results bit for bit.
  * Returns the previous setting. Process-wide. No counterpart in the reference. */
 int zmlhip_set_gemm_lds_dma(int on);
+/* LU panels of more than one workgroup publish column k + 1's pivot
+ * candidates before they finish column k's update (on = 1, the default), or
+ * finish every column first (on = 0). Both give the same factors and pivots
+ * bit for bit. Returns the previous setting. Process-wide. No counterpart in
+ * the reference. */
+int zmlhip_set_leaf_lookahead(int on);
 /* LAPACKE_dgetrs (src/linalg/lapack.zig:1575; lapack/getrs.zig:13-142) */
 int zmlhip_dgetrs(int layout, char trans, int n, int nrhs, const double *a, int lda,
                   const int *ipiv, double *b, int ldb, void *stream);

@@ -81,6 +81,7 @@ SIGNATURES = {
     "zmlhip_set_panel_spin_limit": (_c_int, [_c_long]),
     "zmlhip_set_panel_one_launch_rows": (_c_int, [_c_long]),
     "zmlhip_set_gemm_lds_dma": (_c_int, [_c_int]),
+    "zmlhip_set_leaf_lookahead": (_c_int, [_c_int]),
     "zmlhip_dlaswp": (_c_int, [_c_int, _c_int, _p, _c_int, _c_int, _c_int, _p, _c_int, _p]),
     "zmlhip_dgetrf": (_c_int, [_c_int, _c_int, _c_int, _p, _c_int, _p, _p, _p]),
     "zmlhip_dgetrs": (_c_int, [_c_int, _c_char, _c_int, _c_int, _p, _c_int, _p, _p, _c_int, _p]),

@@ -540,6 +540,12 @@ int zmlhip_set_panel_one_launch_rows(long rows) {
   return 0;
 }
 
+int zmlhip_set_leaf_lookahead(int on) {
+  const int was = leaf_lookahead() ? 1 : 0;
+  set_leaf_lookahead(on);
+  return was;
+}
+
 int zmlhip_set_gemm_lds_dma(int on) {
   const int was = gemm_dma_enabled() ? 1 : 0;
   set_gemm_dma(on);

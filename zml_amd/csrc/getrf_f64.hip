@@ -59,9 +59,12 @@ static LeafState& leaf_state(hipStream_t s, void** ws_out) {
   }
   if (st.max_resident == 0) {
     int per_cu = 0;
+    int per_cu_la = 0;
     ZH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, panel::leaf_kernel<double, true>, panel::pt_of<double>(), 0));
-    st.max_resident = std::max(1, per_cu) * device_cu_count();
+    ZH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu_la, panel::leaf_la_kernel<double>, panel::la_threads<double>(), 0));
+    st.max_resident = std::max(1, std::min(per_cu, per_cu_la)) * device_cu_count();
   }
   *ws_out = ws;
   return st;
@@ -86,8 +89,12 @@ void leaf_factor(int64_t m, T* A, int64_t lda, int64_t r0, int64_t c0, int w, in
     // tags of this launch: base + 1 .. base + 64, above every earlier launch's
     const uint32_t base = (uint32_t)st.launches * (uint32_t)(panel::W + 1);
     st.launches++;
-    hipLaunchKernelGGL((panel::leaf_kernel<T, true>), dim3(G), dim3(PT), 0, s, M, w,
-                       A + r0 * lda + c0, lda, ipiv + r0, r0, info, c0, w_, base, panel_spin_limit());
+    if (leaf_lookahead())
+      hipLaunchKernelGGL((panel::leaf_la_kernel<T>), dim3(G), dim3(panel::la_threads<T>()), 0, s, M, w, A + r0 * lda + c0, lda,
+                         ipiv + r0, r0, info, c0, w_, base, panel_spin_limit());
+    else
+      hipLaunchKernelGGL((panel::leaf_kernel<T, true>), dim3(G), dim3(PT), 0, s, M, w,
+                         A + r0 * lda + c0, lda, ipiv + r0, r0, info, c0, w_, base, panel_spin_limit());
   }
   ZH_LAUNCH_CHECK();
 }

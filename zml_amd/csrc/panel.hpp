@@ -389,6 +389,275 @@ leaf_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restric
   if (g == 0 && t == 0 && first_zero != 0 && *info == 0) *info = (int)(info_base + first_zero);
 }
 
+// ---- look-ahead leaf (more than one workgroup) -------------------------------
+// The leaf LU above with the exchange of column k + 1 started before the
+// update of column k is finished: once column k's pivot row is known, every
+// row applies column k's update to column k + 1 only, the candidates of
+// column k + 1 are chosen and published, and only then the update of columns
+// k + 2 .. W - 1 runs, while the other workgroups' candidates travel. A
+// published row therefore still lacks column k's update in columns > k + 1;
+// its multiplier l (stored at column k) comes with it, and the reader applies
+// a_c - l u_c itself, separately rounded as the owner does, so every value
+// and pivot is the leaf kernel's bit for bit. u and row k are double-buffered
+// by column parity: column k's pivot row is read by the update of its tail
+// after column k + 1's exchange has begun.
+
+// Column k's update of a row, split in two: the head (a[k] <- l, a[k+1] <-
+// a[k+1] - l u[k+1]) and the tail (a[c] <- a[c] - l u[c], c >= k + 2), in the
+// 8-column groups update_row uses, separately rounded.
+template <int G8, typename T>
+__device__ __forceinline__ void head_group(T (&a)[W], T l, const T* __restrict__ u, int k) {
+#pragma clang fp contract(off)
+  constexpr int c0 = 8 * G8;
+  if (c0 + 7 >= k && c0 <= k + 1) {
+#pragma unroll
+    for (int c = c0; c < c0 + 8; ++c) {
+      if (c == k) a[c] = l;
+      else if (c == k + 1) a[c] = a[c] - l * u[c];
+    }
+  }
+}
+template <typename T>
+__device__ __forceinline__ void update_head(T (&a)[W], T l, const T* __restrict__ u, int k) {
+  head_group<0>(a, l, u, k); head_group<1>(a, l, u, k); head_group<2>(a, l, u, k); head_group<3>(a, l, u, k);
+  head_group<4>(a, l, u, k); head_group<5>(a, l, u, k); head_group<6>(a, l, u, k); head_group<7>(a, l, u, k);
+}
+// a[c] <- a[c] - l u[c] for c >= cf (cf uniform), 8 columns at a time
+template <int G8, typename T>
+__device__ __forceinline__ void tail_group(T (&a)[W], T l, const T* __restrict__ u, int cf) {
+#pragma clang fp contract(off)
+  constexpr int c0 = 8 * G8;
+  if (c0 >= cf) {
+#pragma unroll
+    for (int c = c0; c < c0 + 8; ++c) a[c] = a[c] - l * u[c];
+  } else if (c0 + 7 >= cf) {
+#pragma unroll
+    for (int c = c0; c < c0 + 8; ++c)
+      if (c >= cf) a[c] = a[c] - l * u[c];
+  }
+}
+#define ZH_GROUP_FENCE() asm volatile("" ::: "memory")
+template <typename T>
+__device__ __forceinline__ void update_tail(T (&a)[W], T l, const T* __restrict__ u, int cf) {
+  tail_group<0>(a, l, u, cf); ZH_GROUP_FENCE(); tail_group<1>(a, l, u, cf); ZH_GROUP_FENCE();
+  tail_group<2>(a, l, u, cf); ZH_GROUP_FENCE(); tail_group<3>(a, l, u, cf); ZH_GROUP_FENCE();
+  tail_group<4>(a, l, u, cf); ZH_GROUP_FENCE(); tail_group<5>(a, l, u, cf); ZH_GROUP_FENCE();
+  tail_group<6>(a, l, u, cf); ZH_GROUP_FENCE(); tail_group<7>(a, l, u, cf);
+}
+#undef ZH_GROUP_FENCE
+
+// Candidates of column kc (rows >= kc): the wave and workgroup maxima, the
+// candidate row and row kc staged in LDS, then published (records and val).
+template <typename T, int NW>
+__device__ __forceinline__ void la_candidate_publish(int kc, const T (&a)[W], T ak, bool own, int64_t i,
+                                                     int64_t base_row, int g, int t, int lane, int wv, const WS& ws,
+                                                     uint32_t tag_base, T (*stage)[W], T* stage_k, T* s_v,
+                                                     int64_t* s_i, int* s_knan) {
+  constexpr int PT = NW * 64;
+  T v = T(-1);
+  if (own && i >= kc) {
+    const T f = fabs(ak);
+    if (f == f) v = f;
+  }
+  const WaveKey wk = wave_best(WaveKey{(double)v, lane, 0});
+  const int wlane = wk.v >= 0.0 ? wk.idx : -1;
+  if (lane == 0 && wv < NW) {
+    s_v[wv] = (T)wk.v;
+    s_i[wv] = wlane >= 0 ? base_row + wv * 64 + wlane : INT64_MAX;
+  }
+  if (lane == wlane && wv < NW) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) stage[wv][c] = a[c];
+  }
+  if (own && i == kc) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) stage_k[c] = a[c];
+    *s_knan = (ak != ak);
+  }
+  __syncthreads();
+  T bv = s_v[0];
+  int64_t bi = s_i[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) amax(bv, bi, s_v[w], s_i[w]);
+  const int slot = kc & 1;
+  const uint32_t tag = tag_base + (uint32_t)kc + 1u;
+  uint64_t* my_rec = ws.rec + ((size_t)slot * GMAX + g) * REC;
+  uint64_t* rkrec = ws.rowk + (size_t)slot * REC;
+  if (bi != INT64_MAX && wv == (int)((bi - base_row) >> 6)) {
+    st_val<T>(my_rec + 2 + 2 * lane, tag, stage[wv][lane]);
+    if (lane == 0) st64(my_rec, tagged(tag, (uint32_t)bi));
+  }
+  if (kc >= base_row && kc < base_row + PT && wv == (int)((kc - base_row) >> 6)) {
+    st_val<T>(rkrec + 2 + 2 * lane, tag, stage_k[lane]);
+    if (lane == 0) st64(rkrec, tagged(tag, *s_knan ? 1u : 0u));
+  }
+  if (t == 0) st_val<T>(ws.val + ((size_t)slot * GMAX + g) * 2, tag, bi == INT64_MAX ? T(-1) : bv);
+}
+
+// Wave 0: the winner of column kc and row kc, completed with column kc - 1's
+// update, into ub / rkb[kc & 1] and *s_p; *s_abort on a give-up.
+template <typename T>
+__device__ __forceinline__ void la_poll(int kc, int G, int lane, const WS& ws, uint32_t tag_base, long spin_limit,
+                                        T (*ub)[W], T (*rkb)[W], int64_t* s_p, int* s_abort) {
+#pragma clang fp contract(off)
+  const int slot = kc & 1;
+  const uint32_t tag = tag_base + (uint32_t)kc + 1u;
+  const uint64_t* rkrec = ws.rowk + (size_t)slot * REC;
+  if (spin_limit < 0) {   // give-up forced by the host (error-path tests)
+    if (lane == 0) { *s_abort = 1; atomicExch(ws.err, 1); }
+    return;
+  }
+  long spins = 0;
+  T gv = T(-1);
+  int gg = INT32_MAX;
+  while (true) {
+    bool ok = true;
+    gv = T(-1);
+    gg = INT32_MAX;
+    for (int w = lane; w < G; w += 64) {
+      const uint64_t* q = ws.val + ((size_t)slot * GMAX + w) * 2;
+      const uint64_t hi = ld64(q), lo = ld64(q + 1);
+      if (tag_of(hi) != tag || tag_of(lo) != tag) ok = false;
+      else amax(gv, gg, val_of<T>(hi, lo), w);
+    }
+    if (__all(ok)) break;
+    if (++spins > spin_limit) { if (lane == 0) { *s_abort = 1; atomicExch(ws.err, 1); } return; }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  {
+    const WaveKey bk = wave_best(WaveKey{(double)gv, gg, 0});
+    gv = (T)bk.v;
+    gg = bk.idx;
+  }
+  const uint64_t* r = ws.rec + ((size_t)slot * GMAX + (gg == INT32_MAX ? 0 : gg)) * REC;
+  const bool want_r = gv >= T(0);
+  while (true) {
+    const uint64_t rh = ld64(r + 2 + 2 * lane), rl = ld64(r + 3 + 2 * lane);
+    const uint64_t kh = ld64(rkrec + 2 + 2 * lane), kl = ld64(rkrec + 3 + 2 * lane);
+    const uint64_t pw = ld64(r), nw = ld64(rkrec);
+    const bool ok = (!want_r || (tag_of(rh) == tag && tag_of(rl) == tag && tag_of(pw) == tag)) &&
+                    tag_of(kh) == tag && tag_of(kl) == tag && tag_of(nw) == tag;
+    if (__all(ok)) {
+      T rv = val_of<T>(rh, rl), kv = val_of<T>(kh, kl);
+      if (kc > 0) {
+        // both rows still lack column kc - 1's update beyond column kc; their
+        // multipliers sit at column kc - 1
+        const int lc = kc - 1;
+        const T lr = __shfl(rv, lc), lk = __shfl(kv, lc);
+        if (lane > kc) {
+          const T up = ub[lc & 1][lane];
+          rv = rv - lr * up;
+          kv = kv - lk * up;
+        }
+      }
+      const bool nanf = payload(nw) != 0;
+      rkb[slot][lane] = kv;
+      ub[slot][lane] = (nanf || !want_r) ? kv : rv;
+      if (lane == 0) *s_p = (nanf || !want_r) ? (int64_t)kc : (int64_t)payload(pw);
+      return;
+    }
+    if (++spins > spin_limit) { if (lane == 0) { *s_abort = 1; atomicExch(ws.err, 1); } return; }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// threads of a look-ahead leaf workgroup: the row waves and one exchange wave
+template <typename T> constexpr int la_threads() { return pt_of<T>() + 64; }
+
+template <typename T>
+__global__ void __launch_bounds__(la_threads<T>())
+leaf_la_kernel(int64_t M, int Wn, T* __restrict__ P, int64_t ld, int32_t* __restrict__ ipiv,
+               int64_t piv_base, int* __restrict__ info, int64_t info_base, WS ws, uint32_t tag_base,
+               long spin_limit) {
+#pragma clang fp contract(off)
+  constexpr int PT = pt_of<T>();
+  const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  constexpr int NW = PT / 64;
+  // waves 0 .. NW-1 hold the rows; wave NW only exchanges (it polls while
+  // the row waves update the tail of the previous column)
+  const int64_t i = (int64_t)g * PT + t;
+  const bool own = t < PT && i < M;
+  const int64_t base_row = (int64_t)g * PT;
+
+  __shared__ T ub[2][W], rkb[2][W];
+  __shared__ T s_v[NW];
+  __shared__ int64_t s_i[NW];
+  __shared__ int64_t s_p;
+  __shared__ int s_abort, s_knan;
+  __shared__ int32_t s_piv[W];
+  __shared__ T stage[NW][W], stage_k[W];
+
+  T a[W];
+#pragma unroll
+  for (int c = 0; c < W; ++c) a[c] = (own && c < Wn) ? P[i * ld + c] : T(0);
+  if (t == 0) { s_abort = 0; s_knan = 0; }
+
+  int first_zero = 0;
+  const T sfmin = sizeof(T) == 8 ? T(0x1p-1022) : T(0x1p-126);   // lamch('S')
+  T ak = a[0];   // this row's current value in the column being searched
+
+  la_candidate_publish<T, NW>(0, a, ak, own, i, base_row, g, t, lane, wv, ws, tag_base, stage, stage_k, s_v,
+                             s_i, &s_knan);
+  if (wv == NW) la_poll<T>(0, G, lane, ws, tag_base, spin_limit, ub, rkb, &s_p, &s_abort);
+  for (int k = 0; k < Wn; ++k) {
+    __syncthreads();   // column k's pivot row is in, column k - 1's tail updated
+    if (s_abort) {
+      // gave up: the columns not reached keep their rows (identity pivots), so
+      // the interchanges the caller applies next stay inside the matrix
+      if (g == 0 && t < Wn) ipiv[t] = t < k ? s_piv[t] : (int32_t)(piv_base + t + 1);
+      return;
+    }
+    const T* u = ub[k & 1];
+    const T* rk = rkb[k & 1];
+    const int64_t p = s_p;
+    const T piv = u[k];
+    const bool nz = piv != T(0);   // getrf2.zig:55: no interchange on an exact zero
+    if (!nz && first_zero == 0) first_zero = k + 1;
+    if (t == 0) s_piv[k] = (int32_t)(piv_base + (nz ? p : k) + 1);
+    T l = ak;
+    if (nz && p != k) {
+      if (own && i == k) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) a[c] = u[c];
+      } else if (own && i == p) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) a[c] = rk[c];
+        l = rk[k];
+      }
+    }
+    const bool upd = own && i > k;
+    if (upd) {
+      if (nz) {
+        if (fabs(piv) >= sfmin) {
+          const T r = T(1) / piv;
+          l = l * r;
+        } else {
+          l = l / piv;
+        }
+      }
+      update_head<T>(a, l, u, k);
+    }
+    ak = pick<T>(a, k + 1);   // candidate value of the next column
+    if (k + 1 < Wn) {
+      la_candidate_publish<T, NW>(k + 1, a, ak, own, i, base_row, g, t, lane, wv, ws, tag_base, stage, stage_k,
+                                  s_v, s_i, &s_knan);
+      if (wv == NW) la_poll<T>(k + 1, G, lane, ws, tag_base, spin_limit, ub, rkb, &s_p, &s_abort);
+    }
+    if (upd) update_tail<T>(a, l, u, k + 2);
+  }
+
+  // nothing is in flight any more: the row's stores go out back to back
+  __builtin_amdgcn_s_waitcnt(0xF70);
+  if (own) {
+#pragma unroll
+    for (int c = 0; c < W; ++c)
+      if (c < Wn) P[i * ld + c] = a[c];
+  }
+  if (g == 0 && t < Wn) ipiv[t] = s_piv[t];
+  if (g == 0 && t == 0 && first_zero != 0 && *info == 0) *info = (int)(info_base + first_zero);
+}
+
 // ---- row interchanges through the composed permutation ----------------------
 constexpr int PERM_MAX = 1024;  // pairs (2 x block of 512 interchanges)
 

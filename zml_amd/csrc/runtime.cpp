@@ -201,7 +201,10 @@ void set_panel_one_launch_rows(long rows) {
 
 namespace {
 std::atomic<int> g_gemm_dma{1};
+std::atomic<int> g_leaf_la{1};
 }
+bool leaf_lookahead() { return g_leaf_la.load(std::memory_order_relaxed) != 0; }
+void set_leaf_lookahead(int on) { g_leaf_la.store(on ? 1 : 0, std::memory_order_relaxed); }
 bool gemm_dma_enabled() { return g_gemm_dma.load(std::memory_order_relaxed) != 0; }
 void set_gemm_dma(int on) { g_gemm_dma.store(on ? 1 : 0, std::memory_order_relaxed); }
 

@@ -130,6 +130,11 @@ void set_panel_one_launch_rows(long rows);
 // Both kernels give the same results bit for bit.
 bool gemm_dma_enabled();
 void set_gemm_dma(int on);
+// Multi-workgroup LU leaves start the exchange of column k + 1 before the
+// update of column k's tail (panel.hpp leaf_la_kernel; default), or finish
+// each column first (leaf_kernel). Same values and pivots bit for bit.
+bool leaf_lookahead();
+void set_leaf_lookahead(int on);
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
