@@ -251,11 +251,10 @@ __global__ void transpose_f32_kernel(int64_t kb, const float* __restrict__ src, 
 
 // ---- optimistic panel: pivots confined to the diagonal block ----------------------------
 // Partial-pivoting LU of a 256 x 256 FP32 block held in the registers of one
-// workgroup: thread (ty, tx) of a TY x 32 grid holds rows ty + TY i (i < 256 / TY)
-// and columns tx + 32 j (j < 8). Fewer, fuller threads cut the per-column
-// bookkeeping every wave repeats (pivot keys, multipliers). Row interchanges are virtual: a
-// chosen pivot row is marked done and the elimination simply skips done rows;
-// the block is written back in pivot order at the end. ipiv[k] receives the
+// workgroup: thread (ty, tx) of a 32 x 32 grid holds rows ty + 32 i and
+// columns tx + 32 j (i, j < 8). Row interchanges are virtual: a chosen pivot
+// row is marked done and the elimination simply skips done rows; the block is
+// written back in pivot order at the end. ipiv[k] receives the
 // block row that took position k (rowmap_to_ipiv_kernel turns that into the
 // LAPACK form later, off the panel's critical path), and the same order goes
 // out as a list of row moves (destination, source; global rows = block row +
@@ -266,28 +265,59 @@ __global__ void transpose_f32_kernel(int64_t kb, const float* __restrict__ src, 
 // block. l = a * fl(1/p) when |p| >= FLT_MIN, else a / p (the rule of zml's
 // getrf2.zig:65-87). A zero or NaN pivot sets *fail.
 constexpr int DB = 256;
-constexpr int DIAG_TY = 32;   // thread rows of the diagonal-block LU (32 x 32 threads measured fastest)
+constexpr int DIAG_TY = 32;   // thread rows of the diagonal-block LU
 
-__device__ __forceinline__ uint64_t pivot_key(float v, int r) {
-  // larger |a| first, then lower row; NaN below every number, 0 = no candidate
-  if (v != v) return (uint64_t)(0x7fffffffu - (uint32_t)r);
-  return ((uint64_t)__float_as_uint(v) << 32) | (0xffffffffu - (uint32_t)r);
-}
 __device__ __forceinline__ int key_row(uint64_t k) {
   const uint32_t lo = (uint32_t)k;
   return lo >= 0x80000000u ? (int)(0xffffffffu - lo) : (int)(0x7fffffffu - lo);
 }
 
+// The largest int of the wave, in every lane: quad permutations, row rotations
+// by 4 and 8, then the row broadcasts 15 and 31 (lane 63 ends with the
+// maximum), read from lane 63.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp_max_step(int f) {
+  // lanes outside ROW_MASK keep their own value
+  const int o = __builtin_amdgcn_update_dpp(f, f, CTRL, ROW_MASK, 0xf, false);
+  return o > f ? o : f;
+}
+__device__ __forceinline__ int wave64_max(int f) {
+  f = dpp_max_step<0xb1>(f);          // quad_perm [1,0,3,2]
+  f = dpp_max_step<0x4e>(f);          // quad_perm [2,3,0,1]
+  f = dpp_max_step<0x124>(f);         // row_ror:4
+  f = dpp_max_step<0x128>(f);         // row_ror:8
+  f = dpp_max_step<0x142, 0xa>(f);    // row_bcast:15 into rows 1 and 3
+  f = dpp_max_step<0x143, 0xc>(f);    // row_bcast:31 into rows 2 and 3
+  return __builtin_amdgcn_readlane(f, 63);
+}
+
+// The block is eliminated in chunks of 32 columns (one column group). For a
+// chunk, its 256 x 32 entries go to LDS, and four panel waves, one block row
+// per lane, run the chunk's 32 pivot steps on them: each wave ranks its rows
+// in registers and publishes its best row and key, one barrier, every lane
+// takes the winner, forms its multiplier and updates its row's later chunk
+// columns. The other twelve waves wait at the same barriers. Then every
+// thread applies the chunk to its rows' later column groups, step by step:
+// the pivot rows' values there (U12) come from a forward substitution with
+// the chunk's unit lower factor (one column per thread), and row i receives
+// a[i][c] -= l_ik u_k[c] for k in step order while it has not been chosen
+// (multiplier 0 afterwards). Every entry sees the same operations in the same
+// order as in a column-by-column elimination, so the bits are the same.
 template <int TY>
 __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__ P, int64_t ld,
                                                             int32_t* __restrict__ ipiv, int base,
                                                             int* __restrict__ fail, int32_t* __restrict__ pairs,
                                                             int32_t* __restrict__ bpairs, int bappend) {
   constexpr int TX = 32, RI = DB / TY, RJ = DB / TX, NT = TY * TX;
-  static_assert(NT >= DB, "one thread per block row in the epilogue");
-  __shared__ unsigned long long gkey[2];   // pivot keys of consecutive columns
-  __shared__ float colk[DB], urow[DB];
-  __shared__ int prow[DB], invp[DB];
+  constexpr int CW = TX;        // chunk width: one column group
+  constexpr int CS = CW + 4;    // chunk row stride in floats (rows stay 16-byte aligned)
+  constexpr int PW = DB / 64;   // panel waves
+  static_assert(TY == 32 && NT == 1024 && CW % 4 == 0, "32 x 32 threads, 8 x 8 entries each");
+  __shared__ __align__(16) float chunk[DB * CS];
+  __shared__ __align__(16) float ubuf[CW * DB];   // row s: the chunk's pivot s on the later columns
+  __shared__ __align__(16) float cand[2][PW][CW];   // each panel wave's best row, by step parity
+  __shared__ unsigned long long wkey[2][PW];
+  __shared__ int prow[DB], rstep[DB], invp[DB];   // rstep[r]: the step that chose row r, or -1
   __shared__ int nmove[DB / 64], bfirst;
   __shared__ int bad;
   const int t = threadIdx.x, tx = t % TX, ty = t / TX, lane = t & 63, wv = t >> 6;
@@ -296,90 +326,164 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
   for (int i = 0; i < RI; ++i)
 #pragma unroll
     for (int j = 0; j < RJ; ++j) a[i][j] = P[(int64_t)(ty + TY * i) * ld + tx + TX * j];
-  if (t == 0) {
-    bad = 0;
-    gkey[0] = 0;
-  }
-  unsigned done = 0;
-  __syncthreads();
-#pragma unroll
+  if (t < DB) rstep[t] = -1;
+  if (t == 0) bad = 0;
+#pragma unroll 1
   for (int jj = 0; jj < RJ; ++jj) {
-    for (int kq = 0; kq < TX; ++kq) {
-      const int k = TX * jj + kq;
-      // the owners of column k (tx == kq) publish it and their best candidate
-      uint64_t best = 0;
-      if (tx == kq) {
+    const int k0 = CW * jj;
 #pragma unroll
-        for (int i = 0; i < RI; ++i) {
-          colk[ty + TY * i] = a[i][jj];
-          if (!((done >> i) & 1u)) {
-            const uint64_t key = pivot_key(fabsf(a[i][jj]), ty + TY * i);
-            best = key > best ? key : best;
+    for (int j = 0; j < RJ; ++j)
+      if (j == jj) {
+#pragma unroll
+        for (int i = 0; i < RI; ++i) chunk[(ty + TY * i) * CS + tx] = a[i][j];
+      }
+    __syncthreads();
+    if (wv < PW) {
+      // ---- the chunk's pivot steps; lane: block row r
+      const int r = t;
+      float x[CW];
+#pragma unroll
+      for (int q = 0; q < CW / 4; ++q) {
+        const float4 v = *reinterpret_cast<const float4*>(&chunk[r * CS + 4 * q]);
+        x[4 * q] = v.x;
+        x[4 * q + 1] = v.y;
+        x[4 * q + 2] = v.z;
+        x[4 * q + 3] = v.w;
+      }
+      bool rdone = rstep[r] >= 0;
+#pragma unroll
+      for (int kq = 0; kq < CW; ++kq) {
+        const int k = k0 + kq, b = kq & 1;
+        // rank: the bits of |a| (ordered as |a|), then -1 for NaN (below
+        // every number), -2 for a row already chosen
+        const int vb = __float_as_int(x[kq]) & 0x7fffffff;
+        const int f = rdone ? -2 : (vb > 0x7f800000 ? -1 : vb);
+        const int fm = wave64_max(f);
+        if (fm != -2) {
+          // the wave's best row (the lowest of equals) from column kq on, and
+          // its key: larger |a| first, then lower row; NaN below every number
+          const int ls = (int)__builtin_ctzll(__ballot(f == fm));
+          if (lane == ls) {
+#pragma unroll
+            for (int q = kq / 4; q < CW / 4; ++q)
+              *reinterpret_cast<float4*>(&cand[b][wv][4 * q]) =
+                  make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
           }
-        }
-      }
-      // the workgroup's pivot: each wave's best key meets the others in an LDS
-      // maximum (two slots: the next column's is cleared now, its last readers
-      // having passed this column's second barrier)
-      const uint64_t other = __shfl_xor(best, 32);
-      best = other > best ? other : best;
-      if (lane == kq) atomicMax(&gkey[k & 1], (unsigned long long)best);
-      if (t == 0) gkey[(k + 1) & 1] = 0;
-      __syncthreads();
-      uint64_t g = gkey[k & 1];
-      g = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32) |
-          (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)g);
-      const int p = key_row(g), pt = p % TY, pi = p / TY;
-      const float pv = colk[p];
-      const bool ok = pv != 0.0f && pv == pv;
-      if (ty == pt) {   // the pivot row's owners publish the row (its value at k is the pivot)
-#pragma unroll
-        for (int i = 0; i < RI; ++i)
-          if (i == pi) {
-#pragma unroll
-            for (int j = 0; j < RJ; ++j) urow[tx + TX * j] = a[i][j];
+          if (lane == 0) {
+            const uint32_t wrow = (uint32_t)(64 * wv + ls);
+            wkey[b][wv] = fm == -1 ? (unsigned long long)(0x7fffffffu - wrow)
+                                   : ((unsigned long long)(uint32_t)fm << 32) | (0xffffffffu - wrow);
           }
-        done |= 1u << pi;
-      }
-      if (t == 0) {
-        prow[k] = p;
-        if (!ok) bad = 1;
-      }
-      // every thread forms its own rows' multipliers from the published column
-      const bool recip = fabsf(pv) >= 0x1p-126f;
-      const float rp = 1.0f / pv;
-      // (recip is the same in every thread: the division runs only for a subnormal pivot)
-      float li[RI];
-      if (recip) {
-#pragma unroll
-        for (int i = 0; i < RI; ++i) {
-          const float c = colk[ty + TY * i];
-          li[i] = (!((done >> i) & 1u) && ok) ? c * rp : 0.0f;
+        } else if (lane == 0) {
+          wkey[b][wv] = 0;   // no candidate
         }
-      } else {
+        // cand and wkey of this parity are next written at step kq + 2, after
+        // the barrier of step kq + 1, which follows every read of them here
+        __syncthreads();
+        unsigned long long g = wkey[b][0];
 #pragma unroll
-        for (int i = 0; i < RI; ++i) {
-          const float c = colk[ty + TY * i];
-          li[i] = (!((done >> i) & 1u) && ok) ? c / pv : 0.0f;
+        for (int w = 1; w < PW; ++w) {
+          const unsigned long long o = wkey[b][w];
+          g = o > g ? o : g;
         }
-      }
-      __syncthreads();
-      if (tx == kq) {
-#pragma unroll
-        for (int i = 0; i < RI; ++i)
-          if (!((done >> i) & 1u) && ok) a[i][jj] = li[i];
-      }
-#pragma unroll
-      for (int j = jj; j < RJ; ++j) {
-        const float u = urow[tx + TX * j];
-        if (j > jj || tx > kq) {
-#pragma unroll
-          for (int i = 0; i < RI; ++i) a[i][j] -= li[i] * u;
+        g = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)g);
+        const int p = key_row(g), pw = p >> 6;
+        const float pv = cand[b][pw][kq];
+        const bool ok = pv != 0.0f && pv == pv;
+        rdone = rdone || r == p;
+        if (t == 0) {
+          prow[k] = p;
+          rstep[p] = k;
+          if (!ok) bad = 1;
         }
+        // (recip is the same in every lane: the division runs only for a subnormal pivot)
+        const bool recip = fabsf(pv) >= 0x1p-126f;
+        const float rp = 1.0f / pv;
+        float li;
+        if (recip)
+          li = (!rdone && ok) ? x[kq] * rp : 0.0f;
+        else
+          li = (!rdone && ok) ? x[kq] / pv : 0.0f;
+        if (!rdone && ok) x[kq] = li;
+#pragma unroll
+        for (int c = kq + 1; c < CW; ++c) x[c] -= li * cand[b][pw][c];
       }
-      // colk and urow are rewritten only after the next step's first barrier,
-      // which every thread reaches after finishing its reads here
+#pragma unroll
+      for (int q = 0; q < CW / 4; ++q)
+        *reinterpret_cast<float4*>(&chunk[r * CS + 4 * q]) =
+            make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
+    } else {
+      for (int kq = 0; kq < CW; ++kq) __syncthreads();   // the panel waves' step barriers
     }
+    __syncthreads();
+    // ---- the chunk applied to the later column groups
+#pragma unroll
+    for (int j = 0; j < RJ; ++j)
+      if (j == jj) {
+#pragma unroll
+        for (int i = 0; i < RI; ++i) a[i][j] = chunk[(ty + TY * i) * CS + tx];
+      }
+    int rs[RI];
+#pragma unroll
+    for (int i = 0; i < RI; ++i) {
+      rs[i] = rstep[ty + TY * i];
+      const int s = rs[i] - k0;
+      if (s >= 0 && s < CW) {
+#pragma unroll
+        for (int j = 0; j < RJ; ++j)
+          if (j > jj) ubuf[s * DB + tx + TX * j] = a[i][j];
+      }
+    }
+    __syncthreads();
+    if (t < DB - k0 - CW) {
+      // U12 column c: pivot row s minus the earlier pivots' multiples, in step order
+      const int c = k0 + CW + t;
+      float u[CW];
+#pragma unroll
+      for (int s = 0; s < CW; ++s) {
+        const float* ls = &chunk[prow[k0 + s] * CS];
+        float v = ubuf[s * DB + c];
+#pragma unroll
+        for (int q = 0; 4 * q < s; ++q) {
+          const float4 l4 = *reinterpret_cast<const float4*>(ls + 4 * q);
+          const float lv[4] = {l4.x, l4.y, l4.z, l4.w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (4 * q + e < s) v -= lv[e] * u[4 * q + e];
+        }
+        u[s] = v;
+        ubuf[s * DB + c] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int q = 0; q < CW / 4; ++q) {
+      float l4[RI][4];
+#pragma unroll
+      for (int i = 0; i < RI; ++i) {
+        const float4 v = *reinterpret_cast<const float4*>(&chunk[(ty + TY * i) * CS + 4 * q]);
+        l4[i][0] = v.x;
+        l4[i][1] = v.y;
+        l4[i][2] = v.z;
+        l4[i][3] = v.w;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int k = k0 + 4 * q + e;
+        float l[RI];
+#pragma unroll
+        for (int i = 0; i < RI; ++i) l[i] = (rs[i] < 0 || rs[i] > k) ? l4[i][e] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < RJ; ++j)
+          if (j > jj) {
+            const float u = ubuf[(4 * q + e) * DB + tx + TX * j];
+#pragma unroll
+            for (int i = 0; i < RI; ++i) a[i][j] -= l[i] * u;
+          }
+      }
+    }
+    __syncthreads();   // chunk and ubuf are rewritten by the next chunk
   }
   // row r of the block goes to position invp[r]; position k moves in from
   // block row prow[k] when they differ (moves numbered in position order)
