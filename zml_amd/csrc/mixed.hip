@@ -315,8 +315,9 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
   static_assert(TY == 32 && NT == 1024 && CW % 4 == 0, "32 x 32 threads, 8 x 8 entries each");
   __shared__ __align__(16) float chunk[DB * CS];
   __shared__ __align__(16) float ubuf[CW * DB];   // row s: the chunk's pivot s on the later columns
+  __shared__ __align__(16) float lz[DB * CS];   // the multiplier each row took at each step (0 once chosen)
   __shared__ __align__(16) float cand[2][PW][CW];   // each panel wave's best row, by step parity
-  __shared__ unsigned long long wkey[2][PW];
+  __shared__ __align__(16) uint4 wrec[2][PW];   // and its key, value and reciprocal
   __shared__ int prow[DB], rstep[DB], invp[DB];   // rstep[r]: the step that chose row r, or -1
   __shared__ int nmove[DB / 64], bfirst;
   __shared__ int bad;
@@ -359,37 +360,41 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
         const int vb = __float_as_int(x[kq]) & 0x7fffffff;
         const int f = rdone ? -2 : (vb > 0x7f800000 ? -1 : vb);
         const int fm = wave64_max(f);
-        if (fm != -2) {
-          // the wave's best row (the lowest of equals) from column kq on, and
-          // its key: larger |a| first, then lower row; NaN below every number
-          const int ls = (int)__builtin_ctzll(__ballot(f == fm));
-          if (lane == ls) {
+        // the wave's best row (the lowest of equals) from column kq on, and a
+        // record of its key (larger |a| first, then lower row; NaN below every
+        // number; 0 = no candidate), value and reciprocal
+        const int ls = (int)__builtin_ctzll(__ballot(f == fm));
+        if (fm != -2 && lane == ls) {
 #pragma unroll
-            for (int q = kq / 4; q < CW / 4; ++q)
-              *reinterpret_cast<float4*>(&cand[b][wv][4 * q]) =
-                  make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
-          }
-          if (lane == 0) {
-            const uint32_t wrow = (uint32_t)(64 * wv + ls);
-            wkey[b][wv] = fm == -1 ? (unsigned long long)(0x7fffffffu - wrow)
-                                   : ((unsigned long long)(uint32_t)fm << 32) | (0xffffffffu - wrow);
-          }
-        } else if (lane == 0) {
-          wkey[b][wv] = 0;   // no candidate
+          for (int q = kq / 4; q < CW / 4; ++q)
+            *reinterpret_cast<float4*>(&cand[b][wv][4 * q]) =
+                make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
         }
-        // cand and wkey of this parity are next written at step kq + 2, after
+        {
+          const float bv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[kq]), ls));
+          const float brp = 1.0f / bv;
+          const uint32_t wrow = (uint32_t)(64 * wv + ls);
+          const uint64_t key = fm == -2 ? 0ull
+                               : fm == -1 ? (uint64_t)(0x7fffffffu - wrow)
+                                          : ((uint64_t)(uint32_t)fm << 32) | (0xffffffffu - wrow);
+          if (lane == 0)
+            wrec[b][wv] = make_uint4((uint32_t)key, (uint32_t)(key >> 32), __float_as_uint(bv), __float_as_uint(brp));
+        }
+        // cand and wrec of this parity are next written at step kq + 2, after
         // the barrier of step kq + 1, which follows every read of them here
         __syncthreads();
-        unsigned long long g = wkey[b][0];
+        uint4 best = wrec[b][0];
 #pragma unroll
         for (int w = 1; w < PW; ++w) {
-          const unsigned long long o = wkey[b][w];
-          g = o > g ? o : g;
+          const uint4 o = wrec[b][w];
+          const uint64_t ko = ((uint64_t)o.y << 32) | o.x, kb = ((uint64_t)best.y << 32) | best.x;
+          if (ko > kb) best = o;
         }
-        g = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32) |
-            (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)g);
+        const uint64_t g = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(best.y) << 32) |
+                           (uint32_t)__builtin_amdgcn_readfirstlane(best.x);
         const int p = key_row(g), pw = p >> 6;
-        const float pv = cand[b][pw][kq];
+        const float pv = __int_as_float(__builtin_amdgcn_readfirstlane(best.z));
+        const float rp = __int_as_float(__builtin_amdgcn_readfirstlane(best.w));
         const bool ok = pv != 0.0f && pv == pv;
         rdone = rdone || r == p;
         if (t == 0) {
@@ -399,12 +404,12 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
         }
         // (recip is the same in every lane: the division runs only for a subnormal pivot)
         const bool recip = fabsf(pv) >= 0x1p-126f;
-        const float rp = 1.0f / pv;
         float li;
         if (recip)
           li = (!rdone && ok) ? x[kq] * rp : 0.0f;
         else
           li = (!rdone && ok) ? x[kq] / pv : 0.0f;
+        lz[r * CS + kq] = li;
         if (!rdone && ok) x[kq] = li;
 #pragma unroll
         for (int c = kq + 1; c < CW; ++c) x[c] -= li * cand[b][pw][c];
@@ -462,7 +467,7 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
       float l4[RI][4];
 #pragma unroll
       for (int i = 0; i < RI; ++i) {
-        const float4 v = *reinterpret_cast<const float4*>(&chunk[(ty + TY * i) * CS + 4 * q]);
+        const float4 v = *reinterpret_cast<const float4*>(&lz[(ty + TY * i) * CS + 4 * q]);
         l4[i][0] = v.x;
         l4[i][1] = v.y;
         l4[i][2] = v.z;
@@ -470,16 +475,12 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
       }
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int k = k0 + 4 * q + e;
-        float l[RI];
-#pragma unroll
-        for (int i = 0; i < RI; ++i) l[i] = (rs[i] < 0 || rs[i] > k) ? l4[i][e] : 0.0f;
 #pragma unroll
         for (int j = 0; j < RJ; ++j)
           if (j > jj) {
             const float u = ubuf[(4 * q + e) * DB + tx + TX * j];
 #pragma unroll
-            for (int i = 0; i < RI; ++i) a[i][j] -= l[i] * u;
+            for (int i = 0; i < RI; ++i) a[i][j] -= l4[i][e] * u;
           }
       }
     }
