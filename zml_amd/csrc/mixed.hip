@@ -291,33 +291,34 @@ __device__ __forceinline__ int wave64_max(int f) {
   return __builtin_amdgcn_readlane(f, 63);
 }
 
-// The block is eliminated in chunks of 32 columns (one column group). For a
-// chunk, its 256 x 32 entries go to LDS, and four panel waves, one block row
-// per lane, run the chunk's 32 pivot steps on them: each wave ranks its rows
-// in registers and publishes its best row and key, one barrier, every lane
-// takes the winner, forms its multiplier and updates its row's later chunk
-// columns. The other twelve waves wait at the same barriers. Then every
-// thread applies the chunk to its rows' later column groups, step by step:
-// the pivot rows' values there (U12) come from a forward substitution with
-// the chunk's unit lower factor (one column per thread), and row i receives
-// a[i][c] -= l_ik u_k[c] for k in step order while it has not been chosen
-// (multiplier 0 afterwards). Every entry sees the same operations in the same
-// order as in a column-by-column elimination, so the bits are the same.
+// The block is eliminated in chunks of 16 columns (half a column group). One
+// wave, four block rows per lane, runs a chunk's 16 pivot steps in its
+// registers: each row's rank, the wave maximum through DPP, the pivot row
+// read out of the winning lane, the multipliers and the update of the chunk's
+// later columns, with no barrier between steps (its own 64 entries wait in
+// LDS meanwhile). Then the chunk is applied to the later columns, step by
+// step: the pivot rows' values there (U12) come from a forward substitution
+// with the chunk's unit lower factor (one column per thread), and row i
+// receives a[i][c] -= l_ik u_k[c] for k in step order while it has not been
+// chosen (multiplier 0 afterwards). Every entry sees the same operations in
+// the same order as in a column-by-column elimination, so the bits are the
+// same.
 template <int TY>
 __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__ P, int64_t ld,
                                                             int32_t* __restrict__ ipiv, int base,
                                                             int* __restrict__ fail, int32_t* __restrict__ pairs,
                                                             int32_t* __restrict__ bpairs, int bappend) {
   constexpr int TX = 32, RI = DB / TY, RJ = DB / TX, NT = TY * TX;
-  constexpr int CW = TX;        // chunk width: one column group
+  constexpr int CW = 16;        // chunk width: half a column group
   constexpr int CS = CW + 4;    // chunk row stride in floats (rows stay 16-byte aligned)
-  constexpr int PW = DB / 64;   // panel waves
-  static_assert(TY == 32 && NT == 1024 && CW % 4 == 0, "32 x 32 threads, 8 x 8 entries each");
+  constexpr int NC = DB / CW;   // chunks
+  constexpr int RL = DB / 64;   // block rows per lane of the panel wave: lane + 64 m
+  static_assert(TY == 32 && NT == 1024 && TX == 2 * CW, "32 x 32 threads, 8 x 8 entries each");
   __shared__ __align__(16) float chunk[DB * CS];
-  __shared__ __align__(16) float ubuf[CW * DB];   // row s: the chunk's pivot s on the later columns
   __shared__ __align__(16) float lz[DB * CS];   // the multiplier each row took at each step (0 once chosen)
-  __shared__ __align__(16) float cand[2][PW][CW];   // each panel wave's best row, by step parity
-  __shared__ __align__(16) uint4 wrec[2][PW];   // and its key, value and reciprocal
+  __shared__ __align__(16) float ubuf[CW * DB];   // row s: the chunk's pivot s on the later columns
+  __shared__ float park[64 * RI * RJ];   // the panel wave's own entries during the steps
+  __shared__ __align__(16) float prbuf[RL * CW];   // the winning lane's rows at a step
   __shared__ int prow[DB], rstep[DB], invp[DB];   // rstep[r]: the step that chose row r, or -1
   __shared__ int nmove[DB / 64], bfirst;
   __shared__ int bad;
@@ -330,114 +331,147 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
   if (t < DB) rstep[t] = -1;
   if (t == 0) bad = 0;
 #pragma unroll 1
-  for (int jj = 0; jj < RJ; ++jj) {
-    const int k0 = CW * jj;
+  for (int ch = 0; ch < NC; ++ch) {
+    const int k0 = CW * ch, jg = ch >> 1, h = ch & 1;
+    const bool mine = (tx / CW) == h;   // this thread holds chunk columns (register group jg)
+    const int cx = tx % CW;
+    if (mine) {
 #pragma unroll
-    for (int j = 0; j < RJ; ++j)
-      if (j == jj) {
+      for (int j = 0; j < RJ; ++j)
+        if (j == jg) {
 #pragma unroll
-        for (int i = 0; i < RI; ++i) chunk[(ty + TY * i) * CS + tx] = a[i][j];
-      }
+          for (int i = 0; i < RI; ++i) chunk[(ty + TY * i) * CS + cx] = a[i][j];
+        }
+    }
     __syncthreads();
-    if (wv < PW) {
-      // ---- the chunk's pivot steps; lane: block row r
-      const int r = t;
-      float x[CW];
+    if (wv == 0) {
+      // ---- the chunk's pivot steps
 #pragma unroll
-      for (int q = 0; q < CW / 4; ++q) {
-        const float4 v = *reinterpret_cast<const float4*>(&chunk[r * CS + 4 * q]);
-        x[4 * q] = v.x;
-        x[4 * q + 1] = v.y;
-        x[4 * q + 2] = v.z;
-        x[4 * q + 3] = v.w;
+      for (int i = 0; i < RI; ++i)
+#pragma unroll
+        for (int j = 0; j < RJ; ++j) park[(i * RJ + j) * 64 + lane] = a[i][j];
+      float x[RL][CW];
+      bool rd[RL];
+#pragma unroll
+      for (int m = 0; m < RL; ++m) {
+        const int row = lane + 64 * m;
+#pragma unroll
+        for (int q = 0; q < CW / 4; ++q) {
+          const float4 v = *reinterpret_cast<const float4*>(&chunk[row * CS + 4 * q]);
+          x[m][4 * q] = v.x;
+          x[m][4 * q + 1] = v.y;
+          x[m][4 * q + 2] = v.z;
+          x[m][4 * q + 3] = v.w;
+        }
+        rd[m] = rstep[row] >= 0;
       }
-      bool rdone = rstep[r] >= 0;
 #pragma unroll
       for (int kq = 0; kq < CW; ++kq) {
-        const int k = k0 + kq, b = kq & 1;
+        const int k = k0 + kq;
         // rank: the bits of |a| (ordered as |a|), then -1 for NaN (below
         // every number), -2 for a row already chosen
-        const int vb = __float_as_int(x[kq]) & 0x7fffffff;
-        const int f = rdone ? -2 : (vb > 0x7f800000 ? -1 : vb);
-        const int fm = wave64_max(f);
-        // the wave's best row (the lowest of equals) from column kq on, and a
-        // record of its key (larger |a| first, then lower row; NaN below every
-        // number; 0 = no candidate), value and reciprocal
-        const int ls = (int)__builtin_ctzll(__ballot(f == fm));
-        if (fm != -2 && lane == ls) {
+        int f[RL], fl = -2;
 #pragma unroll
-          for (int q = kq / 4; q < CW / 4; ++q)
-            *reinterpret_cast<float4*>(&cand[b][wv][4 * q]) =
-                make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
+        for (int m = 0; m < RL; ++m) {
+          const int vb = __float_as_int(x[m][kq]) & 0x7fffffff;
+          f[m] = rd[m] ? -2 : (vb > 0x7f800000 ? -1 : vb);
+          fl = f[m] > fl ? f[m] : fl;
         }
-        {
-          const float bv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[kq]), ls));
-          const float brp = 1.0f / bv;
-          const uint32_t wrow = (uint32_t)(64 * wv + ls);
-          const uint64_t key = fm == -2 ? 0ull
-                               : fm == -1 ? (uint64_t)(0x7fffffffu - wrow)
-                                          : ((uint64_t)(uint32_t)fm << 32) | (0xffffffffu - wrow);
-          if (lane == 0)
-            wrec[b][wv] = make_uint4((uint32_t)key, (uint32_t)(key >> 32), __float_as_uint(bv), __float_as_uint(brp));
-        }
-        // cand and wrec of this parity are next written at step kq + 2, after
-        // the barrier of step kq + 1, which follows every read of them here
-        __syncthreads();
-        uint4 best = wrec[b][0];
+        const int fm = wave64_max(fl);
+        // the lowest row of rank fm: the first m with a hit, its lowest lane
+        // (some row is always left, so fm >= -1 and a hit exists)
+        int pm = 0;
+        uint64_t hit = 0;
 #pragma unroll
-        for (int w = 1; w < PW; ++w) {
-          const uint4 o = wrec[b][w];
-          const uint64_t ko = ((uint64_t)o.y << 32) | o.x, kb = ((uint64_t)best.y << 32) | best.x;
-          if (ko > kb) best = o;
+        for (int m = RL - 1; m >= 0; --m) {
+          const uint64_t bm = __ballot(f[m] == fm);
+          if (bm) {
+            hit = bm;
+            pm = m;
+          }
         }
-        const uint64_t g = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(best.y) << 32) |
-                           (uint32_t)__builtin_amdgcn_readfirstlane(best.x);
-        const int p = key_row(g), pw = p >> 6;
-        const float pv = __int_as_float(__builtin_amdgcn_readfirstlane(best.z));
-        const float rp = __int_as_float(__builtin_amdgcn_readfirstlane(best.w));
+        const int pl = hit ? (int)__builtin_ctzll(hit) : 0, p = pl + 64 * pm;
+        // the pivot row from column kq on: lane pl puts its rows out, the wave
+        // reads row pm back (the buffer is rewritten only after these reads)
+        if (lane == pl) {
+#pragma unroll
+          for (int m = 0; m < RL; ++m)
+#pragma unroll
+            for (int q = 0; q < CW / 4; ++q)
+              if (q >= kq / 4)
+                *reinterpret_cast<float4*>(&prbuf[m * CW + 4 * q]) =
+                    make_float4(x[m][4 * q], x[m][4 * q + 1], x[m][4 * q + 2], x[m][4 * q + 3]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        float pr[CW];
+#pragma unroll
+        for (int q = 0; q < CW / 4; ++q) {
+          if (q >= kq / 4) {
+            const float4 v = *reinterpret_cast<const float4*>(&prbuf[pm * CW + 4 * q]);
+            pr[4 * q] = v.x;
+            pr[4 * q + 1] = v.y;
+            pr[4 * q + 2] = v.z;
+            pr[4 * q + 3] = v.w;
+          }
+        }
+        const float pv = pr[kq];
         const bool ok = pv != 0.0f && pv == pv;
-        rdone = rdone || r == p;
-        if (t == 0) {
+#pragma unroll
+        for (int m = 0; m < RL; ++m) rd[m] = rd[m] || (m == pm && lane == pl);
+        if (lane == 0) {
           prow[k] = p;
           rstep[p] = k;
           if (!ok) bad = 1;
         }
         // (recip is the same in every lane: the division runs only for a subnormal pivot)
         const bool recip = fabsf(pv) >= 0x1p-126f;
-        float li;
-        if (recip)
-          li = (!rdone && ok) ? x[kq] * rp : 0.0f;
-        else
-          li = (!rdone && ok) ? x[kq] / pv : 0.0f;
-        lz[r * CS + kq] = li;
-        if (!rdone && ok) x[kq] = li;
+        const float rp = 1.0f / pv;
 #pragma unroll
-        for (int c = kq + 1; c < CW; ++c) x[c] -= li * cand[b][pw][c];
+        for (int m = 0; m < RL; ++m) {
+          float li;
+          if (recip)
+            li = (!rd[m] && ok) ? x[m][kq] * rp : 0.0f;
+          else
+            li = (!rd[m] && ok) ? x[m][kq] / pv : 0.0f;
+          lz[(lane + 64 * m) * CS + kq] = li;
+          if (!rd[m] && ok) x[m][kq] = li;
+#pragma unroll
+          for (int c = 0; c < CW; ++c)
+            if (c > kq) x[m][c] -= li * pr[c];
+        }
       }
 #pragma unroll
-      for (int q = 0; q < CW / 4; ++q)
-        *reinterpret_cast<float4*>(&chunk[r * CS + 4 * q]) =
-            make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
-    } else {
-      for (int kq = 0; kq < CW; ++kq) __syncthreads();   // the panel waves' step barriers
+      for (int m = 0; m < RL; ++m)
+#pragma unroll
+        for (int q = 0; q < CW / 4; ++q)
+          *reinterpret_cast<float4*>(&chunk[(lane + 64 * m) * CS + 4 * q]) =
+              make_float4(x[m][4 * q], x[m][4 * q + 1], x[m][4 * q + 2], x[m][4 * q + 3]);
+#pragma unroll
+      for (int i = 0; i < RI; ++i)
+#pragma unroll
+        for (int j = 0; j < RJ; ++j) a[i][j] = park[(i * RJ + j) * 64 + lane];
     }
     __syncthreads();
-    // ---- the chunk applied to the later column groups
+    // ---- the chunk applied to the later columns: the rest of group jg when
+    // the chunk is its first half (tx >= CW), and the groups after jg
+    if (mine) {
 #pragma unroll
-    for (int j = 0; j < RJ; ++j)
-      if (j == jj) {
+      for (int j = 0; j < RJ; ++j)
+        if (j == jg) {
 #pragma unroll
-        for (int i = 0; i < RI; ++i) a[i][j] = chunk[(ty + TY * i) * CS + tx];
-      }
-    int rs[RI];
+          for (int i = 0; i < RI; ++i) a[i][j] = chunk[(ty + TY * i) * CS + cx];
+        }
+    }
+    const bool later_half = h == 0 && tx >= CW;
 #pragma unroll
     for (int i = 0; i < RI; ++i) {
-      rs[i] = rstep[ty + TY * i];
-      const int s = rs[i] - k0;
+      const int s = rstep[ty + TY * i] - k0;
       if (s >= 0 && s < CW) {
 #pragma unroll
         for (int j = 0; j < RJ; ++j)
-          if (j > jj) ubuf[s * DB + tx + TX * j] = a[i][j];
+          if (j > jg || (j == jg && later_half)) ubuf[s * DB + tx + TX * j] = a[i][j];
       }
     }
     __syncthreads();
@@ -477,14 +511,14 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
       for (int e = 0; e < 4; ++e) {
 #pragma unroll
         for (int j = 0; j < RJ; ++j)
-          if (j > jj) {
+          if (j > jg || (j == jg && later_half)) {
             const float u = ubuf[(4 * q + e) * DB + tx + TX * j];
 #pragma unroll
             for (int i = 0; i < RI; ++i) a[i][j] -= l4[i][e] * u;
           }
       }
     }
-    __syncthreads();   // chunk and ubuf are rewritten by the next chunk
+    __syncthreads();   // chunk, lz and ubuf are rewritten by the next chunk
   }
   // row r of the block goes to position invp[r]; position k moves in from
   // block row prow[k] when they differ (moves numbered in position order)
