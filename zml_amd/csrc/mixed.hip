@@ -294,9 +294,9 @@ __device__ __forceinline__ int wave64_max(int f) {
 // The block is eliminated in chunks of 16 columns (half a column group). One
 // wave, four block rows per lane, runs a chunk's 16 pivot steps in its
 // registers: each row's rank, the wave maximum through DPP, the pivot row
-// read out of the winning lane, the multipliers and the update of the chunk's
-// later columns, with no barrier between steps (its own 64 entries wait in
-// LDS meanwhile). Then the chunk is applied to the later columns, step by
+// passed from the winning lane through LDS, the multipliers and the update of
+// the chunk's later columns, with no barrier between steps (the wave's own
+// 64 entries wait in LDS meanwhile). Then the chunk is applied to the later columns, step by
 // step: the pivot rows' values there (U12) come from a forward substitution
 // with the chunk's unit lower factor (one column per thread), and row i
 // receives a[i][c] -= l_ik u_k[c] for k in step order while it has not been
