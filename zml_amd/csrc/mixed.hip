@@ -263,7 +263,8 @@ __global__ void transpose_f32_kernel(int64_t kb, const float* __restrict__ src, 
 // `bappend` (the block's rows follow those of the earlier blocks). Pivot =
 // largest |a| among the rows not yet chosen, ties to the lowest row of the
 // block. l = a * fl(1/p) when |p| >= FLT_MIN, else a / p (the rule of zml's
-// getrf2.zig:65-87). A zero or NaN pivot sets *fail.
+// getrf2.zig:65-87). A zero or NaN pivot sets *fail (which the block column's
+// first block, bappend == 0, overwrites: 1 or 0).
 constexpr int DB = 256;
 constexpr int DIAG_TY = 32;   // thread rows of the diagonal-block LU
 
@@ -531,7 +532,12 @@ __global__ void __launch_bounds__(TY * 32) diag_getf2_kernel(float* __restrict__
   }
   if (t == 0) {
     bfirst = bpairs && bappend ? bpairs[0] : 0;
-    if (bad) atomicOr(fail, 1);
+    // the block column's first block starts the outcome word afresh
+    if (bappend) {
+      if (bad) atomicOr(fail, 1);
+    } else {
+      *fail = bad ? 1 : 0;
+    }
   }
   __syncthreads();
   if (t < DB) {
@@ -1164,11 +1170,17 @@ void panel_product_f32(int64_t m, int64_t n, int64_t K, float alpha, const float
 // `bpairs` when given (appended if `bappend`).
 void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, int64_t src_ld, int32_t* ipiv,
                       int base, float* uinv, int64_t uinv_ld, hipStream_t s, int32_t* pairs,
-                      float* linv = nullptr, int64_t linv_ld = 0, int32_t* bpairs = nullptr, int bappend = 0) {
+                      float* linv = nullptr, int64_t linv_ld = 0, int32_t* bpairs = nullptr, int bappend = 0,
+                      float* linv_init = nullptr) {
   MixedCtx& mx = tl_mix;
   hipLaunchKernelGGL(diag_getf2_kernel<DIAG_TY>, dim3(1), dim3(DIAG_TY * 32), 0, s, P, lda, ipiv, base, mx.pfail,
                      pairs, bpairs, bappend);
   ZH_LAUNCH_CHECK();
+  if (linv_init) {   // the block column's whole L11^-1 buffer starts as the identity
+    hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(linv_ld * linv_ld)), dim3(256), 0, s, linv_ld,
+                       linv_init, linv_ld);
+    ZH_LAUNCH_CHECK();
+  }
   if (linv) {
     const hipStream_t s2 = aux_stream(2);
     ZH_CHECK(hipEventRecord(aux_event(4), s));
@@ -1222,22 +1234,18 @@ bool try_panel_f32(int64_t m, float* A, int64_t lda, int64_t j0, int64_t jb, int
   mx.pending_rows = rows;
   float* P = A + j0 * lda + j0;
   float* ustore = mx.invu_all + (j0 / NB_MIX) * NB_MIX * NB_MIX;   // ld NB_MIX
-  ZH_CHECK(hipMemsetAsync(mx.pfail, 0, sizeof(int), s));
   ZH_CHECK(hipMemcpy2DAsync(mx.pbak, sizeof(float) * jb, P, sizeof(float) * lda, sizeof(float) * jb, rows,
                             hipMemcpyDeviceToDevice, s));
   // L11^-1 (512 x 512, kept for the driver's U12): La^-1 and Lb^-1 beside the
   // halves' other work, the off-diagonal block at the end
   float* linv = jb == 2 * DB ? mx.inv_all + (j0 / NB_MIX) * NB_MIX * NB_MIX : nullptr;
-  if (linv) {
-    hipLaunchKernelGGL(fill_identity_f32_kernel, dim3(blocks(jb * jb)), dim3(256), 0, s, jb, linv, jb);
-    ZH_LAUNCH_CHECK();
-  }
   // row moves: each half's own list, and the block column's (both halves) for
   // the driver, in the buffer of the block column's parity
   int32_t* pr = perm_pairs_buffer(2);
   const int par = (int)((j0 / NB_MIX) & 1);
   int32_t* bpr = jb == NB_MIX ? perm_pairs_buffer(19 + par) : nullptr;
-  optimistic_block(P, lda, rows, mx.pbak + DB * jb, jb, ipiv + j0, (int)j0, ustore, NB_MIX, s, pr, linv, jb, bpr, 0);
+  optimistic_block(P, lda, rows, mx.pbak + DB * jb, jb, ipiv + j0, (int)j0, ustore, NB_MIX, s, pr, linv, jb, bpr, 0,
+                   linv);
   if (jb == 2 * DB) {
     const int64_t c2 = j0 + DB;                 // second half: columns c2 .. c2 + DB
     const int64_t rows2 = rows - DB;
