@@ -278,8 +278,8 @@ __device__ __forceinline__ int key_row(uint64_t k) {
 // maximum), read from lane 63.
 template <int CTRL, int ROW_MASK = 0xf>
 __device__ __forceinline__ int dpp_max_step(int f) {
-  // lanes outside ROW_MASK keep their own value
-  const int o = __builtin_amdgcn_update_dpp(f, f, CTRL, ROW_MASK, 0xf, false);
+  // lanes outside ROW_MASK see INT_MIN and keep their own value
+  const int o = __builtin_amdgcn_update_dpp((int)0x80000000, f, CTRL, ROW_MASK, 0xf, false);
   return o > f ? o : f;
 }
 __device__ __forceinline__ int wave64_max(int f) {
