@@ -831,9 +831,7 @@ struct MixedCtx {
   float tau = 16.0f;         // largest |l| accepted below the diagonal block
   float* pbak = nullptr;     // copy of the block column (rows x NB_MIX)
   float* ptmp = nullptr;     // A21 of the second half (rows x DB)
-  float* scr1 = nullptr;     // DB x DB scratch blocks
-  float* scr2 = nullptr;
-  float* scr3 = nullptr;
+  float* scr3 = nullptr;     // DB x DB scratch blocks
   float* scr4 = nullptr;
   int* pfail = nullptr;
   int panels = 0, panels_fast = 0;
@@ -1193,12 +1191,10 @@ void optimistic_block(float* P, int64_t lda, int64_t rows, const float* src21, i
     hipLaunchKernelGGL(rowmap_to_ipiv_kernel, dim3(1), dim3(DB), 0, s, ipiv, base);
     ZH_LAUNCH_CHECK();
   }
-  // U11^-1 (upper, its lower blocks written as zeros) into scr1
-  tri_inv256(P, lda, mx.scr1, DB, 0, s, true);
-  ZH_CHECK(hipMemcpy2DAsync(uinv, sizeof(float) * uinv_ld, mx.scr1, sizeof(float) * DB, sizeof(float) * DB, DB,
-                            hipMemcpyDeviceToDevice, s));
+  // U11^-1 (upper, its lower blocks written as zeros) straight into its store
+  tri_inv256(P, lda, uinv, uinv_ld, 0, s, true);
   if (rows > DB) {
-    panel_product_f32(rows - DB, DB, DB, 1.0f, src21, src_ld, mx.scr1, DB, 0.0f, P + DB * lda, lda, s);
+    panel_product_f32(rows - DB, DB, DB, 1.0f, src21, src_ld, uinv, uinv_ld, 0.0f, P + DB * lda, lda, s);
     hipLaunchKernelGGL(bound_check_kernel, dim3(blocks((rows - DB) * DB / 16)), dim3(256), 0, s, rows - DB, DB,
                        (const float*)(P + DB * lda), lda, mx.tau, mx.pfail);
     ZH_LAUNCH_CHECK();
@@ -2147,11 +2143,9 @@ int dgesv_mixed(int layout, int64_t n, int64_t nrhs, const double* A, int64_t ld
   };
   {
     char* pw = static_cast<char*>(workspace(
-        sizeof(float) * ((size_t)(n + NB_MIX) * (NB_MIX + DB) + 4 * DB * DB) + 256, 15));
+        sizeof(float) * ((size_t)(n + NB_MIX) * (NB_MIX + DB) + 2 * DB * DB) + 256, 15));
     tl_mix.pbak = reinterpret_cast<float*>(pw); pw += sizeof(float) * (size_t)(n + NB_MIX) * NB_MIX;
     tl_mix.ptmp = reinterpret_cast<float*>(pw); pw += sizeof(float) * (size_t)(n + NB_MIX) * DB;
-    tl_mix.scr1 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
-    tl_mix.scr2 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
     tl_mix.scr3 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
     tl_mix.scr4 = reinterpret_cast<float*>(pw); pw += sizeof(float) * DB * DB;
     tl_mix.pfail = reinterpret_cast<int*>(pw);
